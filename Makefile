@@ -1,0 +1,51 @@
+# MultiBlockMomentsFV on MI355X -- build
+#
+#   make            multiblockmomentsfv_amd/libmbmfv.so (engine + host C) and
+#                   build/moments (drop-in for the reference's ./build/moments)
+#   make oracle     the CPU oracle used by the tests (oracle/liboracle.so)
+#   make clean
+#
+# All HIP code is compiled for gfx950 only.
+
+HIPCC   ?= /opt/rocm/bin/hipcc
+CC      ?= gcc
+ARCH    ?= gfx950
+PKG     := multiblockmomentsfv_amd
+GIT_COMMIT := $(shell git rev-parse --short HEAD 2>/dev/null || echo unknown)
+
+HIPFLAGS := --offload-arch=$(ARCH) -O3 -std=c++17 -fPIC -Wall -Iinclude \
+            -Xclang -target-feature -Xclang -packed-fp32-ops \
+            -Wno-unused-command-line-argument
+CFLAGS   := -O2 -fPIC -Wall -Iinclude -DMBM_GIT_COMMIT=\"$(GIT_COMMIT)\"
+
+LIB     := $(PKG)/libmbmfv.so
+BIN     := build/moments
+OBJDIR  := $(PKG)/_obj
+
+HIPSRC  := $(PKG)/csrc/mbm_engine.hip
+HIPHDR  := $(wildcard $(PKG)/csrc/*.h) include/mbmfv.h include/mbmfv_host.h
+
+.PHONY: all oracle clean
+all: $(LIB) $(BIN)
+
+$(OBJDIR):
+	mkdir -p $(OBJDIR)
+
+$(OBJDIR)/mbm_engine.o: $(HIPSRC) $(HIPHDR) | $(OBJDIR)
+	$(HIPCC) $(HIPFLAGS) -c $< -o $@
+
+$(OBJDIR)/mbm_host.o: $(PKG)/host/mbm_host.c $(HIPHDR) | $(OBJDIR)
+	$(CC) $(CFLAGS) -c $< -o $@
+
+$(LIB): $(OBJDIR)/mbm_engine.o $(OBJDIR)/mbm_host.o
+	$(HIPCC) --offload-arch=$(ARCH) -shared -o $@ $^
+
+$(BIN): $(PKG)/host/moments.c $(LIB) include/mbmfv.h include/mbmfv_host.h
+	mkdir -p build
+	$(CC) $(CFLAGS) -o $@ $< -L$(PKG) -lmbmfv -Wl,-rpath,'$$ORIGIN/../$(PKG)'
+
+oracle:
+	$(MAKE) -C oracle
+
+clean:
+	rm -rf $(OBJDIR) $(LIB) build

@@ -1,0 +1,141 @@
+/*
+ * mbmfv.h -- C ABI of the MI355X (gfx950) engine for the explicit update of
+ * MultiBlockMomentsFV.
+ *
+ * The reference has no plugin or FFI layer: its hot path is a set of plain C
+ * calls inside one process. Each entry point below replaces one of them; the
+ * comment names the reference interface it stands for. Plain C types only:
+ * pointers, sizes and doubles; device pointers are void*. Every function
+ * returns MBM_OK (0) or a negative MBM_E_* code; mbm_last_error() gives the
+ * message of the last failure on the calling thread. A context is used from
+ * one host thread at a time; work is ordered on the context's HIP stream.
+ *
+ * Device layout (per layer = DATA, TMP, NEW): one allocation holding every
+ * block; block k stores 9 component planes of ly*pitch doubles (ly = ny+6,
+ * pitch >= lx = nx+6), cell (a,b) of component m at
+ *   base + off[k] + m*plane[k] + b*pitch[k] + a,
+ * a,b counted from the outermost ghost ring (0-based). FP64 throughout.
+ */
+#ifndef MBMFV_H
+#define MBMFV_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBM_OK 0
+#define MBM_E_ARG -1       /* invalid argument or deck */
+#define MBM_E_NOMEM -2     /* host or device allocation failed */
+#define MBM_E_HIP -3       /* HIP runtime error */
+#define MBM_E_IO -4        /* file I/O */
+#define MBM_E_FORMAT -5    /* malformed block file / output file */
+#define MBM_E_STATE -6     /* call out of order */
+#define MBM_E_NODEV -7     /* no usable gfx950 device */
+
+/* layers (defs.h:21-24) */
+#define MBM_L_DATA 0
+#define MBM_L_TMP 1
+#define MBM_L_NEW 2
+
+typedef struct mbm_ctx mbm_ctx;
+
+/* One block of the block file (Mesh.c:148-179): interior cells, extent and
+ * edge codes left,right,bottom,top (>0 sync with block id, 0 open, -1 wall,
+ * -2 dirichlet). nx, ny >= 3. */
+typedef struct {
+  int nx, ny;
+  double xmin, xmax, ymin, ymax;
+  int bc[4];
+} mbm_block_desc;
+
+/* Physics/numerics parameters: Kn and F (Scenario.c:22-23), the limiter
+ * (Limiter.c:20), the scenario used for the initial state and Dirichlet
+ * ghosts (Scenario.c:72), TempJump parameters {theta_in, theta_out, radius,
+ * width, rho}. */
+typedef struct {
+  double kn;
+  double force;
+  int limiter;
+  int scenario;
+  double scenario_params[5];
+} mbm_params;
+
+/* Per-step record written by mbm_steps (Main.c:199-230 prints these). */
+typedef struct {
+  double cfl;       /* combined CFL of the attempt */
+  double dt;        /* dt used by the attempt */
+  double t;         /* time after the attempt */
+  int accepted;     /* 0 = rejected (CFL > CFLmax) */
+  int step;         /* 1-based step number within this mbm_steps call */
+} mbm_step_record;
+
+const char *mbm_last_error(void);
+
+/* Creates the device state for a deck: three FP64 layers, the ghost map and
+ * the tile list. stream: hipStream_t to order work on, or NULL for the
+ * context's own stream. Replaces main's per-block malloc (Main.c:113-123). */
+int mbm_create(const mbm_block_desc *blocks, int nblocks, const mbm_params *p,
+               void *stream, mbm_ctx **out);
+int mbm_destroy(mbm_ctx *c);                                  /* Main.c:158-161 */
+int mbm_set_stream(mbm_ctx *c, void *stream);
+int mbm_set_params(mbm_ctx *c, const mbm_params *p);          /* CLI overrides, Main.c:97-106 */
+int mbm_set_tile_rows(mbm_ctx *c, int rows);                  /* tile height of the stage kernel */
+
+/* Initial state on interior cells then the ghost fill (Main.c:124-138). */
+int mbm_init(mbm_ctx *c);
+
+/* Host <-> device copies of one block of one layer, AoS (9 doubles per
+ * cell, (ny+6)*(nx+6) cells, x fastest) as the reference's CVector arrays. */
+int mbm_upload(mbm_ctx *c, int layer, int block, const double *host);
+int mbm_download(mbm_ctx *c, int layer, int block, double *host);
+
+/* Device pointer and geometry of one block of one layer (for callers that
+ * keep the state on the device, e.g. a PyTorch wrapper). */
+int mbm_layer_ptr(mbm_ctx *c, int layer, int block, void **dptr,
+                  long long *plane, int *pitch);
+
+/* SetBoundary (Boundary.c:173-193): fills the three ghost rings of every
+ * block of 'layer' at time t, with the reference's corner semantics. */
+int mbm_set_boundary(mbm_ctx *c, int layer, double t);
+
+/* TimeUpdate (Main.c:253-284): stage 1 -> fill(TMP) -> stage 2 -> fill(NEW).
+ * Writes TMP and NEW, leaves DATA untouched, returns the combined CFL
+ * max over blocks and stages of dt*cx/dx + dt*cy/dy + cprod (Main.c:395). */
+int mbm_time_update(mbm_ctx *c, double dt, double t, double *cfl);
+
+/* The New->Data copy (Main.c:236-240) as a buffer rotation: NEW becomes DATA. */
+int mbm_commit(mbm_ctx *c);
+
+/* Steps (Main.c:189-244): adaptive dt from DT_INITIAL=1e-4, rejection while
+ * CFL > cflmax, clamp to tend. Advances *t, returns accepted steps in
+ * *nsteps and rejections in *nrej. If rec != NULL, up to reccap attempt
+ * records are stored and *nrec receives the attempt count. maxsteps > 0
+ * stops after that many accepted steps. dt_io: NULL or *dt_io <= 0 starts
+ * from DT_INITIAL as the reference does on every call; otherwise the run
+ * continues with *dt_io, and *dt_io receives the next dt on return. */
+int mbm_steps(mbm_ctx *c, double *t, double tend, double cflmax, double cflreg,
+              int maxsteps, double *dt_io, mbm_step_record *rec, int reccap,
+              int *nrec, int *nsteps, int *nrej);
+
+/* Output.c:217-243 conversion of DATA for one block, computed on the
+ * device: interior plus one ghost ring, i (x) outer / j (y) inner, values
+ * rho vx vy T p Pxx Pxy Pyy qx qy per point, written to host as float
+ * (float32 != 0) or double. */
+int mbm_output_values(mbm_ctx *c, int block, int float32, void *host);
+
+/* Number of interior cells (tools.c:87-95) and of cells incl. ghosts. */
+long long mbm_interior_cells(const mbm_ctx *c);
+int mbm_block_dims(const mbm_ctx *c, int block, int *nx, int *ny);
+
+/* Stage-kernel timing: mean device time of the last timed stage launches
+ * (ms) recorded with HIP events when timing is enabled. */
+int mbm_enable_timing(mbm_ctx *c, int on);
+int mbm_kernel_time(mbm_ctx *c, double *stage_ms, int *nstage, double *fill_ms,
+                    int *nfill);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

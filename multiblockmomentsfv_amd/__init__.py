@@ -1,0 +1,360 @@
+"""MI355X (gfx950) build of the explicit update of MultiBlockMomentsFV.
+
+Python mirror of the C ABI in ``include/mbmfv.h`` and ``include/mbmfv_host.h``
+(loaded from the in-tree ``libmbmfv.so`` with ctypes).  The engine keeps the
+three FP64 layers DATA/TMP/NEW on the GPU; the host only sees the CFL scalar
+per step and the fields at output times.
+
+    deck = read_deck("cases/square4.txt")
+    eng = Engine.from_deck(deck)
+    eng.init()
+    t, nsteps, nrej, recs = eng.steps(0.0, deck.tout[0], cflreg=deck.cfl)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmbmfv.so")
+MOMENTS_BIN = os.path.join(os.path.dirname(HERE), "build", "moments")
+
+NV = 9
+MBC = 3
+MAX_BLOCKS = 32
+MAX_OUT = 50
+L_DATA, L_TMP, L_NEW = 0, 1, 2
+
+LIMITERS = {"none": 0, "average": 1, "minmod": 2, "superbee": 3, "harmonic": 4}
+LIMITER_NAMES = {v: k for k, v in LIMITERS.items()}
+SCENARIOS = {"riemann2d": 0, "ffs": 1, "tempjump": 2}
+TEMPJUMP_DEFAULT = (5.0, 1.0, 0.4, 0.05, 1.0)
+
+
+class MbmError(RuntimeError):
+    pass
+
+
+class BlockDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+                ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+                ("bc", ctypes.c_int * 4)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("kn", ctypes.c_double), ("force", ctypes.c_double),
+                ("limiter", ctypes.c_int), ("scenario", ctypes.c_int),
+                ("scenario_params", ctypes.c_double * 5)]
+
+
+class StepRecord(ctypes.Structure):
+    _fields_ = [("cfl", ctypes.c_double), ("dt", ctypes.c_double), ("t", ctypes.c_double),
+                ("accepted", ctypes.c_int), ("step", ctypes.c_int)]
+
+
+class DeckStruct(ctypes.Structure):
+    _fields_ = [("nblocks", ctypes.c_int),
+                ("nx", ctypes.c_int * MAX_BLOCKS), ("ny", ctypes.c_int * MAX_BLOCKS),
+                ("xmin", ctypes.c_double * MAX_BLOCKS), ("xmax", ctypes.c_double * MAX_BLOCKS),
+                ("ymin", ctypes.c_double * MAX_BLOCKS), ("ymax", ctypes.c_double * MAX_BLOCKS),
+                ("bc", (ctypes.c_int * 4) * MAX_BLOCKS),
+                ("kn", ctypes.c_double), ("r13", ctypes.c_double),
+                ("force", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("limiter", ctypes.c_int), ("refine", ctypes.c_int),
+                ("nout", ctypes.c_int), ("tout", ctypes.c_double * (MAX_OUT + 1))]
+
+
+class Geom(ctypes.Structure):
+    _fields_ = [("bx", ctypes.c_double), ("by", ctypes.c_double),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libmbmfv.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MbmError(f"{LIB_PATH} is not built: run `make` at the repository root")
+    L = ctypes.CDLL(LIB_PATH)
+    p, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    sigs = {
+        "mbm_last_error": (ctypes.c_char_p, []),
+        "mbm_create": (i, [ctypes.POINTER(BlockDesc), i, ctypes.POINTER(Params), p,
+                           ctypes.POINTER(p)]),
+        "mbm_destroy": (i, [p]),
+        "mbm_set_stream": (i, [p, p]),
+        "mbm_set_params": (i, [p, ctypes.POINTER(Params)]),
+        "mbm_set_tile_rows": (i, [p, i]),
+        "mbm_init": (i, [p]),
+        "mbm_upload": (i, [p, i, i, p]),
+        "mbm_download": (i, [p, i, i, p]),
+        "mbm_layer_ptr": (i, [p, i, i, ctypes.POINTER(p), ctypes.POINTER(ctypes.c_longlong),
+                              ctypes.POINTER(i)]),
+        "mbm_set_boundary": (i, [p, i, d]),
+        "mbm_time_update": (i, [p, d, d, ctypes.POINTER(d)]),
+        "mbm_commit": (i, [p]),
+        "mbm_steps": (i, [p, ctypes.POINTER(d), d, d, d, i, ctypes.POINTER(d),
+                          ctypes.POINTER(StepRecord), i,
+                          ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "mbm_output_values": (i, [p, i, i, p]),
+        "mbm_interior_cells": (ctypes.c_longlong, [p]),
+        "mbm_block_dims": (i, [p, i, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "mbm_enable_timing": (i, [p, i]),
+        "mbm_kernel_time": (i, [p, ctypes.POINTER(d), ctypes.POINTER(i), ctypes.POINTER(d),
+                                ctypes.POINTER(i)]),
+        "mbm_host_set_boundary": (i, [ctypes.POINTER(BlockDesc), i, ctypes.POINTER(Params),
+                                      ctypes.POINTER(p), d]),
+        "mbm_host_last_error": (ctypes.c_char_p, []),
+        "mbm_deck_defaults": (None, [ctypes.POINTER(DeckStruct)]),
+        "mbm_deck_read": (i, [ctypes.c_char_p, ctypes.POINTER(DeckStruct), i, i]),
+        "mbm_deck_check": (i, [ctypes.POINTER(DeckStruct)]),
+        "mbm_deck_geom": (Geom, [ctypes.POINTER(DeckStruct), i]),
+        "mbm_full_grid_size": (i, [ctypes.POINTER(DeckStruct)]),
+        "mbm_limiter_code": (i, [ctypes.c_char_p]),
+        "mbm_limiter_name": (ctypes.c_char_p, [i]),
+        "mbm_convert": (ctypes.c_char_p, [d, ctypes.c_char_p]),
+        "mbm_initial_state": (None, [i, p, d, d, d, p]),
+        "mbm_output_write": (i, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(DeckStruct),
+                                 i, d, i, ctypes.POINTER(p)]),
+        "mbm_restart_read": (i, [ctypes.c_char_p, ctypes.POINTER(DeckStruct), ctypes.POINTER(p),
+                                 ctypes.c_char_p]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names the C headers declare (checked against the library by the tests)."""
+    return [n for n in ("mbm_last_error", "mbm_create", "mbm_destroy", "mbm_set_stream",
+                        "mbm_set_params", "mbm_set_tile_rows", "mbm_init", "mbm_upload",
+                        "mbm_download", "mbm_layer_ptr", "mbm_set_boundary",
+                        "mbm_time_update", "mbm_commit", "mbm_steps", "mbm_output_values",
+                        "mbm_interior_cells", "mbm_block_dims", "mbm_enable_timing",
+                        "mbm_kernel_time")]
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().mbm_last_error()
+        raise MbmError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------- decks
+class Deck:
+    """A parsed block file (Mesh.c:60-184)."""
+
+    def __init__(self, st):
+        self._st = st
+        n = st.nblocks
+        self.nblocks = n
+        self.blocks = [dict(nx=st.nx[k], ny=st.ny[k], xmin=st.xmin[k], xmax=st.xmax[k],
+                            ymin=st.ymin[k], ymax=st.ymax[k], bc=list(st.bc[k]))
+                       for k in range(n)]
+        self.kn, self.r13, self.force, self.cfl = st.kn, st.r13, st.force, st.cfl
+        self.limiter = LIMITER_NAMES[st.limiter]
+        self.refine = st.refine
+        self.tout = [st.tout[i] for i in range(st.nout)]
+
+    def struct(self):
+        st = DeckStruct()
+        lib().mbm_deck_defaults(ctypes.byref(st))
+        st.nblocks = self.nblocks
+        for k, b in enumerate(self.blocks):
+            st.nx[k], st.ny[k] = b["nx"], b["ny"]
+            st.xmin[k], st.xmax[k], st.ymin[k], st.ymax[k] = b["xmin"], b["xmax"], b["ymin"], b["ymax"]
+            for s in range(4):
+                st.bc[k][s] = b["bc"][s]
+        st.kn, st.r13, st.force, st.cfl = self.kn, self.r13, self.force, self.cfl
+        st.limiter = LIMITERS[self.limiter]
+        st.refine = self.refine
+        st.nout = len(self.tout)
+        for i, t in enumerate(self.tout):
+            st.tout[i] = t
+        st.tout[len(self.tout)] = 0.0
+        return st
+
+    def geom(self, k):
+        g = lib().mbm_deck_geom(ctypes.byref(self.struct()), k)
+        return g.bx, g.by, g.dx, g.dy
+
+    def interior_cells(self):
+        return sum(b["nx"] * b["ny"] for b in self.blocks)
+
+
+def read_deck(path, refine=0, verbose=False):
+    st = DeckStruct()
+    rc = lib().mbm_deck_read(path.encode(), ctypes.byref(st), int(refine), int(verbose))
+    if rc != 0:
+        raise MbmError(f"reading {path}: {lib().mbm_host_last_error().decode()}")
+    return Deck(st)
+
+
+def make_deck(blocks, kn=0.001, r13=0.0, force=0.0, cfl=0.95, limiter="harmonic", tout=(0.3,)):
+    st = DeckStruct()
+    lib().mbm_deck_defaults(ctypes.byref(st))
+    d = Deck(st)
+    d.nblocks = len(blocks)
+    d.blocks = [dict(b) for b in blocks]
+    d.kn, d.r13, d.force, d.cfl, d.limiter = kn, r13, force, cfl, limiter
+    d.tout = list(tout)
+    return d
+
+
+# ---------------------------------------------------------------- engine
+class Engine:
+    """Device state of one deck (mbm_create .. mbm_destroy)."""
+
+    def __init__(self, blocks, kn=0.001, force=0.0, limiter="harmonic", scenario="riemann2d",
+                 scenario_params=TEMPJUMP_DEFAULT, stream=None):
+        L = lib()
+        self.blocks = [dict(b) for b in blocks]
+        n = len(blocks)
+        arr = (BlockDesc * n)()
+        for k, b in enumerate(blocks):
+            arr[k].nx, arr[k].ny = b["nx"], b["ny"]
+            arr[k].xmin, arr[k].xmax, arr[k].ymin, arr[k].ymax = b["xmin"], b["xmax"], b["ymin"], b["ymax"]
+            for s in range(4):
+                arr[k].bc[s] = b["bc"][s]
+        self._desc = arr
+        self.params = Params()
+        self.params.kn = kn
+        self.params.force = force
+        self.params.limiter = LIMITERS[limiter] if isinstance(limiter, str) else int(limiter)
+        self.params.scenario = SCENARIOS[scenario] if isinstance(scenario, str) else int(scenario)
+        for i, v in enumerate(scenario_params):
+            self.params.scenario_params[i] = v
+        h = ctypes.c_void_p()
+        _check(L.mbm_create(arr, n, ctypes.byref(self.params),
+                            ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)),
+               "mbm_create")
+        self.h = h
+
+    @classmethod
+    def from_deck(cls, deck, scenario="riemann2d", **kw):
+        return cls(deck.blocks, kn=deck.kn, force=deck.force, limiter=deck.limiter,
+                   scenario=scenario, **kw)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def shape(self, k):
+        return (self.blocks[k]["ny"] + 6, self.blocks[k]["nx"] + 6, NV)
+
+    def set_tile_rows(self, rows):
+        _check(lib().mbm_set_tile_rows(self.h, int(rows)), "mbm_set_tile_rows")
+
+    def init(self):
+        _check(lib().mbm_init(self.h), "mbm_init")
+
+    def upload(self, arrays, layer=L_DATA):
+        for k, a in enumerate(arrays):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            assert a.shape == self.shape(k), (a.shape, self.shape(k))
+            _check(lib().mbm_upload(self.h, layer, k, _ptr(a)), "mbm_upload")
+
+    def download(self, layer=L_DATA):
+        out = []
+        for k in range(len(self.blocks)):
+            a = np.empty(self.shape(k), dtype=np.float64)
+            _check(lib().mbm_download(self.h, layer, k, _ptr(a)), "mbm_download")
+            out.append(a)
+        return out
+
+    def set_boundary(self, layer, t):
+        _check(lib().mbm_set_boundary(self.h, layer, t), "mbm_set_boundary")
+
+    def time_update(self, dt, t):
+        c = ctypes.c_double()
+        _check(lib().mbm_time_update(self.h, dt, t, ctypes.byref(c)), "mbm_time_update")
+        return c.value
+
+    def commit(self):
+        _check(lib().mbm_commit(self.h), "mbm_commit")
+
+    def steps(self, t, tend, cflreg, cflmax=1.0, maxsteps=0, reccap=100000, dt=None):
+        """Steps controller (Main.c:189-244). dt=None starts from DT_INITIAL;
+        otherwise continues with dt. Returns (t, accepted, rejected, log) and
+        keeps the next dt in self.next_dt."""
+        tt = ctypes.c_double(t)
+        dtio = ctypes.c_double(dt if dt else 0.0)
+        recs = (StepRecord * reccap)() if reccap else None
+        nrec, ns, nr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().mbm_steps(self.h, ctypes.byref(tt), tend, cflmax, cflreg, maxsteps,
+                               ctypes.byref(dtio), recs, reccap, ctypes.byref(nrec),
+                               ctypes.byref(ns), ctypes.byref(nr)), "mbm_steps")
+        self.next_dt = dtio.value
+        n = min(nrec.value, reccap)
+        log = np.array([(r.cfl, r.dt, r.t, r.accepted) for r in recs[:n]]) if n else np.zeros((0, 4))
+        return tt.value, ns.value, nr.value, log
+
+    def output_values(self, k, float32=False):
+        n = (self.blocks[k]["nx"] + 2) * (self.blocks[k]["ny"] + 2)
+        v = np.empty((n, 10), dtype=np.float32 if float32 else np.float64)
+        _check(lib().mbm_output_values(self.h, k, int(bool(float32)), _ptr(v)), "mbm_output_values")
+        return v
+
+    def interior_cells(self):
+        return lib().mbm_interior_cells(self.h)
+
+    def enable_timing(self, on=True):
+        _check(lib().mbm_enable_timing(self.h, int(on)), "mbm_enable_timing")
+
+    def kernel_time(self):
+        sm, fm = ctypes.c_double(), ctypes.c_double()
+        ns, nf = ctypes.c_int(), ctypes.c_int()
+        _check(lib().mbm_kernel_time(self.h, ctypes.byref(sm), ctypes.byref(ns), ctypes.byref(fm),
+                                     ctypes.byref(nf)), "mbm_kernel_time")
+        return sm.value, ns.value, fm.value, nf.value
+
+
+def host_set_boundary(blocks, arrays, kn=0.001, limiter="harmonic", scenario="riemann2d",
+                      scenario_params=TEMPJUMP_DEFAULT, t=0.0):
+    """CPU application of the engine's ghost map (no GPU needed)."""
+    L = lib()
+    n = len(blocks)
+    arr = (BlockDesc * n)()
+    for k, b in enumerate(blocks):
+        arr[k].nx, arr[k].ny = b["nx"], b["ny"]
+        arr[k].xmin, arr[k].xmax, arr[k].ymin, arr[k].ymax = b["xmin"], b["xmax"], b["ymin"], b["ymax"]
+        for s in range(4):
+            arr[k].bc[s] = b["bc"][s]
+    par = Params()
+    par.kn = kn
+    par.limiter = LIMITERS[limiter]
+    par.scenario = SCENARIOS[scenario]
+    for i, v in enumerate(scenario_params):
+        par.scenario_params[i] = v
+    ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in arrays])
+    for a in arrays:
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    _check(L.mbm_host_set_boundary(arr, n, ctypes.byref(par), ptrs, t), "mbm_host_set_boundary")
+
+
+def initial_state(scenario, x, y, t=0.0, params=TEMPJUMP_DEFAULT):
+    U = np.empty(NV)
+    p = np.asarray(params, dtype=np.float64)
+    lib().mbm_initial_state(SCENARIOS[scenario], _ptr(p), x, y, t, _ptr(U))
+    return U
+
+
+def convert(a):
+    buf = ctypes.create_string_buffer(32)
+    return lib().mbm_convert(a, buf).decode()

@@ -1,0 +1,1071 @@
+/*
+ * mbm_engine.hip -- gfx950 engine for the explicit update of
+ * MultiBlockMomentsFV: stage kernel, ghost-fill kernel, CFL finalisation,
+ * initial state and output conversion, behind the C ABI of mbmfv.h.
+ *
+ * Stage kernel (one Heun stage, Main.c:297-396, fused):
+ *   one wavefront owns a strip of 60 columns (64 lanes: 2 halo lanes on each
+ *   side) and a run of rows of one block. It marches upward row by row,
+ *   keeping the primitives of rows r, r+1, the upper reconstructed state of
+ *   row r, the partial update of row r and the initial value of row r+1 in
+ *   registers. x-neighbours are exchanged with cross-lane moves, so there is
+ *   no LDS and no barrier; every wavefront is independent. Each x-face and
+ *   each y-face is evaluated once. Face speeds and the production CFL are
+ *   reduced per wavefront and folded into per-block maxima with 64-bit
+ *   atomics (non-negative doubles order like their bit patterns).
+ */
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cstdarg>
+#include <vector>
+#include <string>
+#include <algorithm>
+
+#include "mbmfv.h"
+#include "mbmfv_host.h"
+#include "mbm_phys.h"
+#include "mbm_scenario.h"
+
+namespace {
+
+thread_local char g_err[512];
+
+int set_err(int code, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(expr)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return set_err(MBM_E_HIP, "%s failed: %s (%s:%d)", #expr,              \
+                     hipGetErrorString(e_), __FILE__, __LINE__);               \
+  } while (0)
+
+constexpr int NV = 9;
+constexpr int MBC = 3;
+constexpr int STRIP = 60;          /* owned columns per wavefront */
+constexpr int WAVES_PER_WG = 4;
+
+struct BlockDev {
+  long long off;      /* element offset of block in a layer buffer */
+  long long plane;    /* elements between components */
+  int lx, ly, pitch, pad;
+  double dx, dy, bx, by;
+};
+
+struct TileDev {
+  int blk;
+  int a0;             /* column of lane 0 */
+  int r0, r1;         /* first/last iteration row */
+};
+
+/* one ghost cell: value = sign * source, source is a cell of the same layer
+ * or the scenario state at (x,y) */
+struct GhostDev {
+  int dblk, dcell;    /* destination block, b*pitch+a */
+  int sblk, scell;    /* source block/cell, sblk < 0: scenario */
+  int sign;           /* bit m set: component m negated */
+  int pad;
+  double x, y;
+};
+
+struct StageArgs {
+  const double *in;   /* stage 1: DATA, stage 2: TMP */
+  const double *in0;  /* stage 2: DATA */
+  double *out;        /* stage 1: TMP, stage 2: NEW */
+  const BlockDev *blocks;
+  const TileDev *tiles;
+  unsigned long long *slots;   /* per block: max x-face speed, y-face speed, prod CFL */
+  int ntiles;
+  double dt, dtkn, force;
+};
+
+} // namespace
+
+/* ------------------------------------------------------------------ */
+/* cross-lane helpers                                                  */
+/* ------------------------------------------------------------------ */
+namespace mbm {
+
+MBM_DEV double from_left(double v) { return __shfl_up(v, 1, 64); }
+MBM_DEV double from_right(double v) { return __shfl_down(v, 1, 64); }
+
+MBM_DEV double wave_max(double v)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+MBM_DEV void fold_max(double &m, double v) { m = v > m ? v : m; }
+
+/* Loads the 9 components of one cell (raw, no arithmetic). */
+MBM_DEV void load9(const double *__restrict__ p, long long plane, double *U)
+{
+#pragma unroll
+  for (int m = 0; m < NV; m++) U[m] = p[m * plane];
+}
+
+/* Projection of one loaded cell (Main.c:314-336):
+ *   P  = primitives used for slopes and faces: relaxed with flag 1 (stage 1)
+ *        or 3 (stage 2) on rings <= 2, unrelaxed on ring 3;
+ *   I  = initial value of the new state: relaxation flag 0 (stage 1) or
+ *        flag 2 of the stage input plus flag -1 of DATA (stage 2);
+ *   cp = production CFL 0.5*(1-R0(2z/3)) (stage 1 only). */
+template <int STAGE, bool NEED_INIT>
+MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
+                     double dtkn, double force, double *P, double *I, double &cp)
+{
+  double Q[NV];
+  primitive(U, Q);
+  const double z = Q[0] * sqrt(Q[3]) * dtkn;
+  double i1, i2;
+  damp_inv(z, i1, i2);
+  if constexpr (STAGE == 1) {
+    relax(Q, damp1(z, i1, i2), P);
+    if constexpr (NEED_INIT) {
+      const Damp d0 = damp0(z, i1, i2);
+      double R[NV];
+      relax(Q, d0, R);
+      R[1] += dt * force;
+      conservative(R, I);
+      cp = 0.5 * (1.0 - d0.d2);
+    }
+  } else {
+    Damp d3;
+    d3.d1 = i1;
+    d3.d2 = i2;
+    relax(Q, d3, P);
+    if constexpr (NEED_INIT) {
+      double R[NV], Q0[NV], V[NV];
+#pragma unroll
+      for (int n = 0; n < NV; n++) R[n] = Q[n];
+      R[1] += dt * force;
+      conservative(R, I);
+      primitive(U0, Q0);
+      const double z0 = Q0[0] * sqrt(Q0[3]) * dtkn;
+      double j1, j2;
+      damp_inv(z0, j1, j2);
+      relax(Q0, damp0(z0, j1, j2), R);
+      conservative(R, V);
+#pragma unroll
+      for (int n = 0; n < NV; n++) I[n] += V[n];
+      cp = 0.0;
+    }
+  }
+  if (ring3) {
+#pragma unroll
+    for (int n = 0; n < NV; n++) P[n] = Q[n];
+  }
+}
+
+} // namespace mbm
+
+using namespace mbm;
+
+template <int STAGE, int LIM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) stage_kernel(const StageArgs A)
+{
+  const int lane = threadIdx.x & 63;
+  const int tix = blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6);
+  if (tix >= A.ntiles) return;
+  const TileDev T = A.tiles[tix];
+  const BlockDev B = A.blocks[T.blk];
+  const int lx = B.lx, ly = B.ly;
+  const int a = T.a0 + lane;
+  const int ac = a < lx ? a : lx - 1;
+  const long long pl = B.plane;
+  const double *__restrict__ inb = A.in + B.off + ac;
+  const double *__restrict__ in0b = (STAGE == 2) ? A.in0 + B.off + ac : nullptr;
+  double *__restrict__ outb = A.out + B.off + a;
+  const double dt = A.dt, dtkn = A.dtkn, force = A.force;
+  const double dtdx = dt / B.dx, dtdy = dt / B.dy;
+
+  const bool col_ring3 = (a == 0 || a >= lx - 1);
+  const bool col_prod = (a >= 1 && a <= lx - 2);
+  const bool col_face = (a >= 1 && a <= lx - 3);
+  const bool xface_ok = col_face && lane >= 1 && lane <= 61;
+  const bool st_col = (lane >= 2 && lane <= 61) && a >= MBC && a <= lx - 1 - MBC;
+
+  double mxx = 0.0, mxy = 0.0, mxp = 0.0;
+  double U[NV], U0[NV], dummy[NV];
+  double cp = 0.0;
+
+  auto row_of = [&](int r) { return r < 0 ? 0 : (r > ly - 1 ? ly - 1 : r); };
+
+  /* ---- warm-up: rows r0-2 .. r0+1 ---- */
+  const int r0 = T.r0;
+  double Pa[NV], Pb[NV], Pc[NV], Pn[NV], In1[NV], acc[NV], Tup[NV];
+  {
+    int rr = row_of(r0 - 2);
+    load9(inb + (long long)rr * B.pitch, pl, U);
+    project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
+    rr = row_of(r0 - 1);
+    load9(inb + (long long)rr * B.pitch, pl, U);
+    project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp);
+    rr = row_of(r0);
+    load9(inb + (long long)rr * B.pitch, pl, U);
+    if constexpr (STAGE == 2) load9(in0b + (long long)rr * B.pitch, pl, U0);
+    project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp);
+    if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
+    rr = row_of(r0 + 1);
+    load9(inb + (long long)rr * B.pitch, pl, U);
+    if constexpr (STAGE == 2) load9(in0b + (long long)rr * B.pitch, pl, U0);
+    project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, In1, cp);
+    if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
+
+    double Tprev[NV], Bot[NV], G[NV], s;
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+      const double sp = limit<LIM>(Pb[n] - Pa[n], Pc[n] - Pb[n]);
+      Tprev[n] = Pb[n] + 0.5 * sp;
+      const double sc = limit<LIM>(Pc[n] - Pb[n], Pn[n] - Pc[n]);
+      Tup[n] = Pc[n] + 0.5 * sc;
+      Bot[n] = Pc[n] - 0.5 * sc;
+    }
+    face<1>(Tprev, Bot, dtdy, G, s);
+    if (col_face && r0 - 1 >= 1 && r0 - 1 <= ly - 3) fold_max(mxy, s);
+#pragma unroll
+    for (int n = 0; n < NV; n++) acc[n] += G[n];
+  }
+
+  /* ---- march ---- */
+  for (int r = r0; r <= T.r1; r++) {
+    /* prefetch row r+2 (r+2 <= ly-1 since r <= ly-3) */
+    const long long roff = (long long)(r + 2) * B.pitch;
+    load9(inb + roff, pl, U);
+    if constexpr (STAGE == 2) load9(in0b + roff, pl, U0);
+
+    /* x-faces of row r */
+    {
+      double L[NV], Rr[NV], F[NV], s;
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        const double pm = from_left(Pc[n]);
+        const double pp = from_right(Pc[n]);
+        const double sx = limit<LIM>(Pc[n] - pm, pp - Pc[n]);
+        L[n] = Pc[n] + 0.5 * sx;
+        Rr[n] = from_right(Pc[n] - 0.5 * sx);
+      }
+      face<0>(L, Rr, dtdx, F, s);
+      if (xface_ok) fold_max(mxx, s);
+#pragma unroll
+      for (int n = 0; n < NV; n++) acc[n] += from_left(F[n]) - F[n];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+
+    /* row r+2: projection */
+    double P2[NV], I2[NV];
+    const int rr = r + 2;
+    project<STAGE, true>(U, U0, col_ring3 || rr == ly - 1, dt, dtkn, force, P2, I2, cp);
+    if (STAGE == 1 && col_prod && rr <= ly - 2) fold_max(mxp, cp);
+    __builtin_amdgcn_sched_barrier(0);
+
+    /* y-face r+1/2 */
+    {
+      double Tn[NV], Bn[NV], G[NV], s;
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        const double sy = limit<LIM>(Pn[n] - Pc[n], P2[n] - Pn[n]);
+        Tn[n] = Pn[n] + 0.5 * sy;
+        Bn[n] = Pn[n] - 0.5 * sy;
+      }
+      face<1>(Tup, Bn, dtdy, G, s);
+      if (col_face) fold_max(mxy, s);
+      const bool st = st_col && r >= MBC && r <= ly - 1 - MBC;
+      double *o = outb + (long long)r * B.pitch;
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        const double v = acc[n] - G[n];
+        if (st) o[n * pl] = (STAGE == 2) ? 0.5 * v : v;
+        acc[n] = In1[n] + G[n];
+        In1[n] = I2[n];
+        Tup[n] = Tn[n];
+        Pc[n] = Pn[n];
+        Pn[n] = P2[n];
+      }
+    }
+  }
+
+  /* ---- CFL maxima of this wavefront ---- */
+  mxx = wave_max(mxx);
+  mxy = wave_max(mxy);
+  if (STAGE == 1) mxp = wave_max(mxp);
+  if (lane == 0) {
+    unsigned long long *s = A.slots + 3 * T.blk;
+    atomicMax(s + 0, (unsigned long long)__double_as_longlong(mxx));
+    atomicMax(s + 1, (unsigned long long)__double_as_longlong(mxy));
+    if (STAGE == 1) atomicMax(s + 2, (unsigned long long)__double_as_longlong(mxp));
+  }
+}
+
+/* SetBoundary as a gather over the precomputed ghost map (Boundary.c:173-193) */
+__global__ void __launch_bounds__(256) fill_kernel(double *layer, const BlockDev *blocks,
+                                                   const GhostDev *map, int n, int scenario,
+                                                   const double *tj)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const GhostDev g = map[i];
+  const BlockDev D = blocks[g.dblk];
+  double v[NV];
+  if (g.sblk >= 0) {
+    const BlockDev S = blocks[g.sblk];
+    const double *src = layer + S.off + g.scell;
+#pragma unroll
+    for (int m = 0; m < NV; m++) v[m] = src[m * S.plane];
+  } else {
+    double tjl[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) tjl[m] = tj[m];
+    mbm_scn_eval(scenario, tjl, g.x, g.y, v);
+  }
+  double *dst = layer + D.off + g.dcell;
+#pragma unroll
+  for (int m = 0; m < NV; m++) dst[m * D.plane] = ((g.sign >> m) & 1) ? -v[m] : v[m];
+}
+
+/* Combined CFL (Main.c:395 per block, Main.c:268-282 over blocks and stages),
+ * evaluated in the reference's order; clears the slots for the next step. */
+__global__ void finalize_kernel(unsigned long long *slots, const BlockDev *blocks, int nb,
+                                double dt, double *out)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double c1 = 0.0, c2 = 0.0;
+  for (int st = 0; st < 2; st++)
+    for (int k = 0; k < nb; k++) {
+      unsigned long long *s = slots + (size_t)3 * (st * nb + k);
+      const double cx = __longlong_as_double((long long)s[0]);
+      const double cy = __longlong_as_double((long long)s[1]);
+      const double cp = __longlong_as_double((long long)s[2]);
+      const double v = dt * cx / blocks[k].dx + dt * cy / blocks[k].dy + cp;
+      if (st == 0) { if (v > c1) c1 = v; }
+      else { if (v > c2) c2 = v; }
+      s[0] = 0ull; s[1] = 0ull; s[2] = 0ull;
+    }
+  const double m2 = c2 > 0.0 ? c2 : 0.0;
+  out[0] = c1 > m2 ? c1 : m2;
+}
+
+__global__ void zero_kernel(unsigned long long *p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+/* interior initial state (Main.c:126-136) */
+__global__ void init_kernel(double *layer, const BlockDev *blocks, int blk, int nx, int ny,
+                            int scenario, const double *tj)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx * ny) return;
+  const BlockDev B = blocks[blk];
+  const int a = MBC + i % nx, b = MBC + i / nx;
+  double tjl[5], U[NV];
+  for (int m = 0; m < 5; m++) tjl[m] = tj[m];
+  /* GetPoint (tools.c:97-104) without contraction */
+  const double x = __dadd_rn(B.bx, __dmul_rn((double)a, B.dx));
+  const double y = __dadd_rn(B.by, __dmul_rn((double)b, B.dy));
+  mbm_scn_eval(scenario, tjl, x, y, U);
+  double *d = layer + B.off + (long long)b * B.pitch + a;
+  for (int m = 0; m < NV; m++) d[m * B.plane] = U[m];
+}
+
+/* Output.c:217-243: interior + 1 ring, point p = (a-2)*(ny+2) + (b-2) */
+template <typename OUT>
+__global__ void output_kernel(const double *layer, const BlockDev *blocks, int blk, int nx,
+                              int ny, OUT *out)
+{
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ni = ny + 2;
+  if (p >= (nx + 2) * ni) return;
+  const BlockDev B = blocks[blk];
+  const int a = MBC - 1 + p / ni, b = MBC - 1 + p % ni;
+  const double *s = layer + B.off + (long long)b * B.pitch + a;
+  double U[NV];
+  for (int m = 0; m < NV; m++) U[m] = s[m * B.plane];
+  /* Primitive with true divisions (Flux.c:30-56), output only */
+  const double rho = U[0], v1 = U[1] / rho, v2 = U[2] / rho;
+  const double vs = v1 * v1 + v2 * v2;
+  const double th = (2 * U[3] / rho - vs) / 3;
+  const double t11 = U[4] / rho - v1 * v1, t12 = U[5] / rho - v1 * v2, t22 = U[6] / rho - v2 * v2;
+  const double q1 = U[7] - U[3] * v1 - rho * (t11 * v1 + t12 * v2);
+  const double q2 = U[8] - U[3] * v2 - rho * (t12 * v1 + t22 * v2);
+  OUT *o = out + (size_t)10 * p;
+  o[0] = (OUT)rho; o[1] = (OUT)v1; o[2] = (OUT)v2; o[3] = (OUT)th;
+  o[4] = (OUT)(rho * th); o[5] = (OUT)(rho * t11); o[6] = (OUT)(rho * t12);
+  o[7] = (OUT)(rho * t22); o[8] = (OUT)q1; o[9] = (OUT)q2;
+}
+
+/* ------------------------------------------------------------------ */
+/* host side                                                           */
+/* ------------------------------------------------------------------ */
+struct mbm_ctx {
+  int nb;
+  std::vector<mbm_block_desc> desc;
+  std::vector<BlockDev> hblk;
+  mbm_params par;
+  size_t layer_elems;
+  double *layer[3];           /* physical buffers */
+  int role[3];                /* role[L] = physical buffer holding layer L */
+  BlockDev *dblk;
+  TileDev *dtiles;
+  int ntiles;
+  int tile_rows;
+  GhostDev *dghost;
+  int nghost;
+  unsigned long long *dslots; /* 2 stages x nb x 3 */
+  double *dcfl;               /* combined CFL */
+  double *dtj;                /* scenario params */
+  double *hcfl;               /* pinned */
+  hipStream_t stream;
+  bool own_stream;
+  bool timing;
+  hipEvent_t ev[4];
+  double stage_ms, fill_ms;
+  int nstage, nfill;
+  int device;
+};
+
+extern "C" const char *mbm_last_error(void) { return g_err; }
+
+namespace {
+
+/* ghost cell resolution: returns the final value source of ghost (a,b) of
+ * block k after both sweeps of SetBoundary (Boundary.c:145-193). */
+struct Src {
+  int blk, a, b;   /* source cell, blk < 0: scenario at ghost (a,b) of block -blk-1 */
+  int sign;
+};
+
+inline int kind_of(int code) { return code > 0 ? 0 : code == 0 ? 1 : code == -1 ? 2 : 3; }
+
+const int kWallSign[4] = {
+  (1 << 1) | (1 << 5) | (1 << 7),     /* left: flip U1,U5,U7 (Scenario.c:57-63) */
+  (1 << 1) | (1 << 5) | (1 << 7),
+  (1 << 2) | (1 << 5) | (1 << 8),     /* bottom: flip U2,U5,U8 */
+  (1 << 2) | (1 << 5) | (1 << 8) };
+
+void inner_ab(const mbm_block_desc &B, int side, int ti, int s, int &a, int &b)
+{
+  const int lx = B.nx + 2 * MBC, ly = B.ny + 2 * MBC;
+  switch (side) {
+    case 0: a = MBC - 1 + s; b = ti; break;
+    case 1: a = lx - MBC - s; b = ti; break;
+    case 2: a = ti; b = MBC - 1 + s; break;
+    default: a = ti; b = ly - MBC - s; break;
+  }
+}
+
+Src resolve(const std::vector<mbm_block_desc> &D, int k, int a, int b, int depth);
+
+/* non-corner ghost on 'side' at tangential ti, layer n (FillEdge, Boundary.c:88-132) */
+Src resolve_edge(const std::vector<mbm_block_desc> &D, int k, int side, int ti, int n, int ga,
+                 int gb, int depth)
+{
+  static const int opp[4] = { 1, 0, 3, 2 };
+  const int code = D[k].bc[side];
+  Src s{ k, 0, 0, 0 };
+  switch (kind_of(code)) {
+    case 0: {
+      const int o = code - 1;
+      inner_ab(D[o], opp[side], ti, n, s.a, s.b);
+      s.blk = o;
+      const int lxo = D[o].nx + 2 * MBC, lyo = D[o].ny + 2 * MBC;
+      const bool inner = s.a >= MBC && s.a < lxo - MBC && s.b >= MBC && s.b < lyo - MBC;
+      if (!inner) return resolve(D, o, s.a, s.b, depth + 1);
+      return s;
+    }
+    case 1: inner_ab(D[k], side, ti, 1, s.a, s.b); return s;
+    case 2: inner_ab(D[k], side, ti, n, s.a, s.b); s.sign = kWallSign[side]; return s;
+    default: s.blk = -k - 1; s.a = ga; s.b = gb; return s;
+  }
+}
+
+Src resolve(const std::vector<mbm_block_desc> &D, int k, int a, int b, int depth)
+{
+  static const int opp[4] = { 1, 0, 3, 2 };
+  const int lx = D[k].nx + 2 * MBC, ly = D[k].ny + 2 * MBC;
+  if (depth > 4) return Src{ k, a, b, 0 };  /* unreachable for nx,ny >= 3 */
+  const int xs = a < MBC ? 0 : (a >= lx - MBC ? 1 : -1);
+  const int ys = b < MBC ? 2 : (b >= ly - MBC ? 3 : -1);
+  const int nxg = xs == 0 ? MBC - a : a - (lx - 1 - MBC);
+  const int nyg = ys == 2 ? MBC - b : b - (ly - 1 - MBC);
+  if (xs < 0 && ys < 0) return Src{ k, a, b, 0 };
+  if (ys < 0) return resolve_edge(D, k, xs, b, nxg, a, b, depth);
+  if (xs < 0) return resolve_edge(D, k, ys, a, nyg, a, b, depth);
+  /* corner: last writer of the two sweeps */
+  const int cy = D[k].bc[ys], cx = D[k].bc[xs];
+  int sa, sb;
+  if (kind_of(cy) == 0) {
+    const int o = cy - 1;
+    inner_ab(D[o], opp[ys], a, nyg, sa, sb);
+    return resolve(D, o, sa, sb, depth + 1);
+  }
+  if (kind_of(cx) == 0) {
+    const int o = cx - 1;
+    inner_ab(D[o], opp[xs], b, nxg, sa, sb);
+    return resolve(D, o, sa, sb, depth + 1);
+  }
+  switch (kind_of(cy)) {
+    case 1:
+      inner_ab(D[k], ys, a, 1, sa, sb);
+      return resolve(D, k, sa, sb, depth + 1);
+    case 2: {
+      inner_ab(D[k], ys, a, nyg, sa, sb);
+      Src s = resolve(D, k, sa, sb, depth + 1);
+      s.sign ^= kWallSign[ys];
+      return s;
+    }
+    default:
+      return Src{ -k - 1, a, b, 0 };
+  }
+}
+
+int validate(const mbm_block_desc *blocks, int nb, const mbm_params *p)
+{
+  if (nb < 1 || nb > MBM_MAX_BLOCKS) return set_err(MBM_E_ARG, "nblocks %d outside [1,%d]", nb, MBM_MAX_BLOCKS);
+  if (!p) return set_err(MBM_E_ARG, "params is NULL");
+  if (p->limiter < 0 || p->limiter > 4) return set_err(MBM_E_ARG, "limiter %d invalid", p->limiter);
+  if (p->scenario < 0 || p->scenario > 2) return set_err(MBM_E_ARG, "scenario %d invalid", p->scenario);
+  for (int k = 0; k < nb; k++) {
+    const mbm_block_desc &B = blocks[k];
+    if (B.nx < MBC || B.ny < MBC)
+      return set_err(MBM_E_ARG, "block %d: grid %dx%d smaller than 3x3", k + 1, B.nx, B.ny);
+    if ((long long)(B.nx + 6) * (B.ny + 6) * NV >= (1ll << 31))
+      return set_err(MBM_E_ARG, "block %d too large", k + 1);
+    if (!(B.xmax > B.xmin) || !(B.ymax > B.ymin))
+      return set_err(MBM_E_ARG, "block %d has an empty extent", k + 1);
+    for (int s = 0; s < 4; s++) {
+      const int c = B.bc[s];
+      if (c > nb) return set_err(MBM_E_ARG, "block %d side %d syncs with missing block %d", k + 1, s, c);
+      if (c > 0) {
+        const mbm_block_desc &O = blocks[c - 1];
+        const bool conf = s < 2 ? O.ny == B.ny : O.nx == B.nx;
+        if (!conf)
+          return set_err(MBM_E_ARG, "non-conforming sync interface: block %d side %d vs block %d", k + 1, s, c);
+      }
+    }
+  }
+  return MBM_OK;
+}
+
+/* geometry exactly as SetBlocks (Mesh.c:169-172) */
+void geometry(const mbm_block_desc &B, double &bx, double &by, double &dx, double &dy)
+{
+  const int nst = 2 * MBC;
+  dx = (B.xmax - B.xmin) / B.nx;
+  dy = (B.ymax - B.ymin) / B.ny;
+  bx = B.xmin - (nst / 2 - 0.5) * dx;
+  by = B.ymin - (nst / 2 - 0.5) * dy;
+}
+
+std::vector<GhostDev> build_ghost_map(const std::vector<mbm_block_desc> &D,
+                                      const std::vector<BlockDev> &hb)
+{
+  std::vector<GhostDev> m;
+  for (int k = 0; k < (int)D.size(); k++) {
+    const int lx = D[k].nx + 2 * MBC, ly = D[k].ny + 2 * MBC;
+    for (int b = 0; b < ly; b++)
+      for (int a = 0; a < lx; a++) {
+        const bool ghost = a < MBC || a >= lx - MBC || b < MBC || b >= ly - MBC;
+        if (!ghost) continue;
+        const Src s = resolve(D, k, a, b, 0);
+        GhostDev g{};
+        g.dblk = k;
+        g.dcell = b * hb[k].pitch + a;
+        g.sign = s.sign;
+        if (s.blk >= 0) {
+          g.sblk = s.blk;
+          g.scell = s.b * hb[s.blk].pitch + s.a;
+        } else {
+          const int kk = -s.blk - 1;
+          g.sblk = -1;
+          g.scell = 0;
+          g.x = hb[kk].bx + s.a * hb[kk].dx;
+          g.y = hb[kk].by + s.b * hb[kk].dy;
+        }
+        m.push_back(g);
+      }
+  }
+  return m;
+}
+
+std::vector<TileDev> build_tiles(const std::vector<mbm_block_desc> &D, int rows)
+{
+  std::vector<TileDev> t;
+  for (int k = 0; k < (int)D.size(); k++) {
+    const int lx = D[k].nx + 2 * MBC, ly = D[k].ny + 2 * MBC;
+    /* strip j: lanes [60j, 60j+63], x-faces [60j+1, 60j+61]; need 60j+61 >= lx-3 */
+    for (int a0 = 0;; a0 += STRIP) {
+      for (int r0 = 1; r0 <= ly - 3; r0 += rows) {
+        TileDev d;
+        d.blk = k;
+        d.a0 = a0;
+        d.r0 = r0;
+        d.r1 = std::min(r0 + rows - 1, ly - 3);
+        t.push_back(d);
+      }
+      if (a0 + 61 >= lx - 3) break;
+    }
+  }
+  return t;
+}
+
+int upload_tiles(mbm_ctx *c)
+{
+  std::vector<TileDev> t = build_tiles(c->desc, c->tile_rows);
+  if (c->dtiles) HIPCHK(hipFree(c->dtiles));
+  c->dtiles = nullptr;
+  HIPCHK(hipMalloc(&c->dtiles, t.size() * sizeof(TileDev)));
+  HIPCHK(hipMemcpy(c->dtiles, t.data(), t.size() * sizeof(TileDev), hipMemcpyHostToDevice));
+  c->ntiles = (int)t.size();
+  return MBM_OK;
+}
+
+template <int STAGE>
+int launch_stage(mbm_ctx *c, const StageArgs &A)
+{
+  const dim3 grid((A.ntiles + WAVES_PER_WG - 1) / WAVES_PER_WG), block(64 * WAVES_PER_WG);
+  switch (c->par.limiter) {
+    case 0: hipLaunchKernelGGL((stage_kernel<STAGE, 0>), grid, block, 0, c->stream, A); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<STAGE, 1>), grid, block, 0, c->stream, A); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<STAGE, 2>), grid, block, 0, c->stream, A); break;
+    case 3: hipLaunchKernelGGL((stage_kernel<STAGE, 3>), grid, block, 0, c->stream, A); break;
+    default: hipLaunchKernelGGL((stage_kernel<STAGE, 4>), grid, block, 0, c->stream, A); break;
+  }
+  HIPCHK(hipGetLastError());
+  return MBM_OK;
+}
+
+int do_fill(mbm_ctx *c, int layer)
+{
+  if (c->nghost == 0) return MBM_OK;
+  const int nblk = (c->nghost + 255) / 256;
+  hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, c->layer[c->role[layer]],
+                     c->dblk, c->dghost, c->nghost, c->par.scenario, c->dtj);
+  HIPCHK(hipGetLastError());
+  return MBM_OK;
+}
+
+int upload_params(mbm_ctx *c)
+{
+  HIPCHK(hipMemcpyAsync(c->dtj, c->par.scenario_params, 5 * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream));
+  return MBM_OK;
+}
+
+} // namespace
+
+extern "C" int mbm_create(const mbm_block_desc *blocks, int nblocks, const mbm_params *p,
+                          void *stream, mbm_ctx **out)
+{
+  if (!out || !blocks) return set_err(MBM_E_ARG, "NULL argument");
+  *out = nullptr;
+  int rc = validate(blocks, nblocks, p);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_err(MBM_E_NODEV, "no HIP device available");
+  mbm_ctx *c = new mbm_ctx();
+  c->nb = nblocks;
+  c->desc.assign(blocks, blocks + nblocks);
+  c->par = *p;
+  c->tile_rows = 32;
+  HIPCHK(hipGetDevice(&c->device));
+  size_t off = 0;
+  for (int k = 0; k < nblocks; k++) {
+    BlockDev b{};
+    b.lx = blocks[k].nx + 2 * MBC;
+    b.ly = blocks[k].ny + 2 * MBC;
+    b.pitch = (b.lx + 7) & ~7;
+    b.plane = (long long)b.ly * b.pitch;
+    b.plane = (b.plane + 31) & ~31ll;
+    b.off = (long long)off;
+    geometry(blocks[k], b.bx, b.by, b.dx, b.dy);
+    off += (size_t)b.plane * NV;
+    c->hblk.push_back(b);
+  }
+  c->layer_elems = off;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+  } else {
+    HIPCHK(hipStreamCreate(&c->stream));
+    c->own_stream = true;
+  }
+  for (int l = 0; l < 3; l++) {
+    if (hipMalloc(&c->layer[l], off * sizeof(double)) != hipSuccess) {
+      mbm_destroy(c);
+      return set_err(MBM_E_NOMEM, "device allocation of %zu bytes failed", off * sizeof(double));
+    }
+    HIPCHK(hipMemsetAsync(c->layer[l], 0, off * sizeof(double), c->stream));
+    c->role[l] = l;
+  }
+  HIPCHK(hipMalloc(&c->dblk, nblocks * sizeof(BlockDev)));
+  HIPCHK(hipMemcpy(c->dblk, c->hblk.data(), nblocks * sizeof(BlockDev), hipMemcpyHostToDevice));
+  std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
+  c->nghost = (int)gm.size();
+  HIPCHK(hipMalloc(&c->dghost, std::max<size_t>(1, gm.size()) * sizeof(GhostDev)));
+  HIPCHK(hipMemcpy(c->dghost, gm.data(), gm.size() * sizeof(GhostDev), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&c->dslots, (size_t)6 * nblocks * sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(c->dslots, 0, (size_t)6 * nblocks * sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMalloc(&c->dcfl, sizeof(double)));
+  HIPCHK(hipMalloc(&c->dtj, 5 * sizeof(double)));
+  HIPCHK(hipHostMalloc(&c->hcfl, sizeof(double)));
+  for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+  rc = upload_params(c);
+  if (!rc) rc = upload_tiles(c);
+  if (rc) { mbm_destroy(c); return rc; }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = c;
+  return MBM_OK;
+}
+
+extern "C" int mbm_destroy(mbm_ctx *c)
+{
+  if (!c) return MBM_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int l = 0; l < 3; l++) if (c->layer[l]) (void)hipFree(c->layer[l]);
+  if (c->dblk) (void)hipFree(c->dblk);
+  if (c->dtiles) (void)hipFree(c->dtiles);
+  if (c->dghost) (void)hipFree(c->dghost);
+  if (c->dslots) (void)hipFree(c->dslots);
+  if (c->dcfl) (void)hipFree(c->dcfl);
+  if (c->dtj) (void)hipFree(c->dtj);
+  if (c->hcfl) (void)hipHostFree(c->hcfl);
+  for (int i = 0; i < 4; i++) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MBM_OK;
+}
+
+extern "C" int mbm_set_stream(mbm_ctx *c, void *stream)
+{
+  if (!c) return set_err(MBM_E_ARG, "NULL context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->own_stream) HIPCHK(hipStreamDestroy(c->stream));
+  if (stream) { c->stream = (hipStream_t)stream; c->own_stream = false; }
+  else { HIPCHK(hipStreamCreate(&c->stream)); c->own_stream = true; }
+  return MBM_OK;
+}
+
+extern "C" int mbm_set_params(mbm_ctx *c, const mbm_params *p)
+{
+  if (!c) return set_err(MBM_E_ARG, "NULL context");
+  int rc = validate(c->desc.data(), c->nb, p);
+  if (rc) return rc;
+  c->par = *p;
+  return upload_params(c);
+}
+
+extern "C" int mbm_set_tile_rows(mbm_ctx *c, int rows)
+{
+  if (!c) return set_err(MBM_E_ARG, "NULL context");
+  if (rows < 1 || rows > 4096) return set_err(MBM_E_ARG, "tile rows %d outside [1,4096]", rows);
+  c->tile_rows = rows;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return upload_tiles(c);
+}
+
+extern "C" int mbm_init(mbm_ctx *c)
+{
+  if (!c) return set_err(MBM_E_ARG, "NULL context");
+  c->role[0] = 0; c->role[1] = 1; c->role[2] = 2;
+  for (int l = 0; l < 3; l++)
+    HIPCHK(hipMemsetAsync(c->layer[l], 0, c->layer_elems * sizeof(double), c->stream));
+  for (int k = 0; k < c->nb; k++) {
+    const int n = c->desc[k].nx * c->desc[k].ny;
+    hipLaunchKernelGGL(init_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                       c->layer[c->role[MBM_L_DATA]], c->dblk, k, c->desc[k].nx, c->desc[k].ny,
+                       c->par.scenario, c->dtj);
+    HIPCHK(hipGetLastError());
+  }
+  int rc = do_fill(c, MBM_L_DATA);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MBM_OK;
+}
+
+namespace {
+int check_lb(mbm_ctx *c, int layer, int block)
+{
+  if (!c) return set_err(MBM_E_ARG, "NULL context");
+  if (layer < 0 || layer > 2) return set_err(MBM_E_ARG, "layer %d invalid", layer);
+  if (block < 0 || block >= c->nb) return set_err(MBM_E_ARG, "block %d invalid", block);
+  return MBM_OK;
+}
+} // namespace
+
+extern "C" int mbm_upload(mbm_ctx *c, int layer, int block, const double *host)
+{
+  int rc = check_lb(c, layer, block);
+  if (rc) return rc;
+  const BlockDev &B = c->hblk[block];
+  std::vector<double> soa((size_t)B.plane * NV, 0.0);
+  for (int b = 0; b < B.ly; b++)
+    for (int a = 0; a < B.lx; a++)
+      for (int m = 0; m < NV; m++)
+        soa[(size_t)m * B.plane + (size_t)b * B.pitch + a] = host[(size_t)NV * ((size_t)b * B.lx + a) + m];
+  HIPCHK(hipMemcpyAsync(c->layer[c->role[layer]] + B.off, soa.data(), soa.size() * sizeof(double),
+                        hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MBM_OK;
+}
+
+extern "C" int mbm_download(mbm_ctx *c, int layer, int block, double *host)
+{
+  int rc = check_lb(c, layer, block);
+  if (rc) return rc;
+  const BlockDev &B = c->hblk[block];
+  std::vector<double> soa((size_t)B.plane * NV);
+  HIPCHK(hipMemcpyAsync(soa.data(), c->layer[c->role[layer]] + B.off, soa.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < B.ly; b++)
+    for (int a = 0; a < B.lx; a++)
+      for (int m = 0; m < NV; m++)
+        host[(size_t)NV * ((size_t)b * B.lx + a) + m] = soa[(size_t)m * B.plane + (size_t)b * B.pitch + a];
+  return MBM_OK;
+}
+
+extern "C" int mbm_layer_ptr(mbm_ctx *c, int layer, int block, void **dptr, long long *plane,
+                             int *pitch)
+{
+  int rc = check_lb(c, layer, block);
+  if (rc) return rc;
+  const BlockDev &B = c->hblk[block];
+  if (dptr) *dptr = c->layer[c->role[layer]] + B.off;
+  if (plane) *plane = B.plane;
+  if (pitch) *pitch = B.pitch;
+  return MBM_OK;
+}
+
+extern "C" int mbm_set_boundary(mbm_ctx *c, int layer, double t)
+{
+  int rc = check_lb(c, layer, 0);
+  if (rc) return rc;
+  (void)t; /* the scenarios are steady: Dirichlet values equal the initial state */
+  rc = do_fill(c, layer);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MBM_OK;
+}
+
+namespace {
+
+/* enqueues one TimeUpdate; the combined CFL lands in c->dcfl */
+int enqueue_time_update(mbm_ctx *c, double dt)
+{
+  StageArgs A{};
+  A.blocks = c->dblk;
+  A.tiles = c->dtiles;
+  A.ntiles = c->ntiles;
+  A.dt = dt;
+  A.dtkn = dt / c->par.kn;
+  A.force = c->par.force;
+  A.in = c->layer[c->role[MBM_L_DATA]];
+  A.in0 = A.in;
+  A.out = c->layer[c->role[MBM_L_TMP]];
+  A.slots = c->dslots;
+  int rc;
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  if ((rc = launch_stage<1>(c, A))) return rc;
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  if ((rc = do_fill(c, MBM_L_TMP))) return rc;
+  A.in = c->layer[c->role[MBM_L_TMP]];
+  A.in0 = c->layer[c->role[MBM_L_DATA]];
+  A.out = c->layer[c->role[MBM_L_NEW]];
+  A.slots = c->dslots + 3 * c->nb;
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->stream));
+  if ((rc = launch_stage<2>(c, A))) return rc;
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[3], c->stream));
+  if ((rc = do_fill(c, MBM_L_NEW))) return rc;
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, c->stream, c->dslots, c->dblk, c->nb,
+                     dt, c->dcfl);
+  HIPCHK(hipGetLastError());
+  return MBM_OK;
+}
+
+int read_cfl(mbm_ctx *c, double *cfl)
+{
+  HIPCHK(hipMemcpyAsync(c->hcfl, c->dcfl, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->timing) {
+    float ms1 = 0.f, ms2 = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms1, c->ev[0], c->ev[1]));
+    HIPCHK(hipEventElapsedTime(&ms2, c->ev[2], c->ev[3]));
+    c->stage_ms += ms1 + ms2;
+    c->nstage += 2;
+  }
+  *cfl = *c->hcfl;
+  return MBM_OK;
+}
+
+} // namespace
+
+extern "C" int mbm_time_update(mbm_ctx *c, double dt, double t, double *cfl)
+{
+  if (!c || !cfl) return set_err(MBM_E_ARG, "NULL argument");
+  (void)t;
+  int rc = enqueue_time_update(c, dt);
+  if (rc) return rc;
+  return read_cfl(c, cfl);
+}
+
+extern "C" int mbm_commit(mbm_ctx *c)
+{
+  if (!c) return set_err(MBM_E_ARG, "NULL context");
+  std::swap(c->role[MBM_L_DATA], c->role[MBM_L_NEW]);
+  return MBM_OK;
+}
+
+extern "C" int mbm_steps(mbm_ctx *c, double *t, double tend, double cflmax, double cflreg,
+                         int maxsteps, double *dt_io, mbm_step_record *rec, int reccap, int *nrec,
+                         int *nsteps, int *nrej)
+{
+  if (!c || !t) return set_err(MBM_E_ARG, "NULL argument");
+  int nstep = 0, rej = 0, nr = 0, rc;
+  double cfl, dt = (dt_io && *dt_io > 0.0) ? *dt_io : 0.0001;
+  while (*t < tend) {
+    if (maxsteps > 0 && nstep >= maxsteps) break;
+    nstep++;
+    do {
+      const double dt_used = dt;
+      if ((rc = mbm_time_update(c, dt, *t, &cfl))) return rc;
+      const int acc = !(cfl > cflmax);
+      if (!acc) {
+        dt = dt * cflreg / cfl;
+        rej++;
+      } else {
+        *t += dt;
+        dt = dt * cflreg / cfl;
+        if (*t + dt > tend) dt = tend - (*t);
+      }
+      if (rec && nr < reccap) {
+        rec[nr].cfl = cfl;
+        rec[nr].dt = dt_used;
+        rec[nr].t = *t;
+        rec[nr].accepted = acc;
+        rec[nr].step = nstep;
+      }
+      nr++;
+    } while (cfl > cflmax);
+    mbm_commit(c);
+  }
+  if (nrec) *nrec = nr;
+  if (nsteps) *nsteps = nstep;
+  if (nrej) *nrej = rej;
+  if (dt_io) *dt_io = dt;
+  return MBM_OK;
+}
+
+extern "C" int mbm_output_values(mbm_ctx *c, int block, int float32, void *host)
+{
+  int rc = check_lb(c, 0, block);
+  if (rc) return rc;
+  const int nx = c->desc[block].nx, ny = c->desc[block].ny;
+  const size_t npt = (size_t)(nx + 2) * (ny + 2);
+  const size_t bytes = npt * 10 * (float32 ? sizeof(float) : sizeof(double));
+  void *d = nullptr;
+  HIPCHK(hipMalloc(&d, bytes));
+  const int nblk = (int)((npt + 255) / 256);
+  if (float32)
+    hipLaunchKernelGGL(output_kernel<float>, dim3(nblk), dim3(256), 0, c->stream,
+                       c->layer[c->role[MBM_L_DATA]], c->dblk, block, nx, ny, (float *)d);
+  else
+    hipLaunchKernelGGL(output_kernel<double>, dim3(nblk), dim3(256), 0, c->stream,
+                       c->layer[c->role[MBM_L_DATA]], c->dblk, block, nx, ny, (double *)d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return set_err(MBM_E_HIP, "output conversion failed: %s", hipGetErrorString(e));
+  return MBM_OK;
+}
+
+extern "C" long long mbm_interior_cells(const mbm_ctx *c)
+{
+  if (!c) return 0;
+  long long s = 0;
+  for (int k = 0; k < c->nb; k++) s += (long long)c->desc[k].nx * c->desc[k].ny;
+  return s;
+}
+
+extern "C" int mbm_block_dims(const mbm_ctx *c, int block, int *nx, int *ny)
+{
+  if (!c || block < 0 || block >= c->nb) return set_err(MBM_E_ARG, "invalid block");
+  if (nx) *nx = c->desc[block].nx;
+  if (ny) *ny = c->desc[block].ny;
+  return MBM_OK;
+}
+
+extern "C" int mbm_enable_timing(mbm_ctx *c, int on)
+{
+  if (!c) return set_err(MBM_E_ARG, "NULL context");
+  c->timing = on != 0;
+  c->stage_ms = 0.0;
+  c->nstage = 0;
+  c->fill_ms = 0.0;
+  c->nfill = 0;
+  return MBM_OK;
+}
+
+extern "C" int mbm_kernel_time(mbm_ctx *c, double *stage_ms, int *nstage, double *fill_ms,
+                               int *nfill)
+{
+  if (!c) return set_err(MBM_E_ARG, "NULL context");
+  if (stage_ms) *stage_ms = c->stage_ms;
+  if (nstage) *nstage = c->nstage;
+  if (fill_ms) *fill_ms = c->fill_ms;
+  if (nfill) *nfill = c->nfill;
+  return MBM_OK;
+}
+
+/* CPU application of the ghost map, for checking the map against the
+ * reference semantics without a GPU. layers[k]: AoS block k, (ny+6)*(nx+6)
+ * cells x fastest. */
+extern "C" int mbm_host_set_boundary(const mbm_block_desc *blocks, int nblocks,
+                                     const mbm_params *p, double *const *layers, double t)
+{
+  (void)t;
+  int rc = validate(blocks, nblocks, p);
+  if (rc) return rc;
+  std::vector<mbm_block_desc> D(blocks, blocks + nblocks);
+  std::vector<BlockDev> hb;
+  for (int k = 0; k < nblocks; k++) {
+    BlockDev b{};
+    b.lx = blocks[k].nx + 2 * MBC;
+    b.ly = blocks[k].ny + 2 * MBC;
+    b.pitch = b.lx;
+    geometry(blocks[k], b.bx, b.by, b.dx, b.dy);
+    hb.push_back(b);
+  }
+  std::vector<GhostDev> gm = build_ghost_map(D, hb);
+  std::vector<double> vals(gm.size() * NV);
+  for (size_t i = 0; i < gm.size(); i++) {
+    const GhostDev &g = gm[i];
+    double v[NV];
+    if (g.sblk >= 0) {
+      for (int m = 0; m < NV; m++) v[m] = layers[g.sblk][(size_t)NV * g.scell + m];
+    } else {
+      mbm_scn_eval(p->scenario, p->scenario_params, g.x, g.y, v);
+    }
+    for (int m = 0; m < NV; m++) vals[i * NV + m] = ((g.sign >> m) & 1) ? -v[m] : v[m];
+  }
+  for (size_t i = 0; i < gm.size(); i++)
+    for (int m = 0; m < NV; m++) layers[gm[i].dblk][(size_t)NV * gm[i].dcell + m] = vals[i * NV + m];
+  return MBM_OK;
+}

@@ -1,0 +1,274 @@
+/*
+ * mbm_phys.h -- device-side cell and face physics of the explicit update,
+ * FP64, written for throughput on gfx950's FP64 VALU.
+ *
+ * Same mathematics as the reference (Flux.c, Limiter.c, Scenario.c), in an
+ * algebraically equivalent form with fewer divides:
+ *  - one reciprocal of rho per primitive recovery (Flux.c:30-56);
+ *  - the Grad-13 flux is evaluated from primitives without the
+ *    Conservative -> Primitive round trip of Flux.c:79-119 (rho*M_ijk = c*q,
+ *    so the flux needs no division at all);
+ *  - the relaxation factors R(z), R(2z/3) of Flux.c:213-223 share their
+ *    denominators (1+z/2)^2 and (1+z/3)^2, so both come from one reciprocal;
+ *  - the HLL combine (Flux.c:364-376) uses one reciprocal per face and
+ *    folds dt/dx into it;
+ *  - the slope division by dx (Main.c:346-349) and the 0.5*dx factor of the
+ *    reconstruction (Main.c:360-374) cancel and are dropped.
+ * Results agree with the reference to rounding (<= 1e-12 relative L-inf per
+ * field on the checked decks); NaN propagation is not reproduced.
+ */
+#ifndef MBM_PHYS_H
+#define MBM_PHYS_H
+
+#include <hip/hip_runtime.h>
+
+#define MBM_DEV __device__ __forceinline__
+
+namespace mbm {
+
+/* reciprocal: hardware estimate + two Newton steps (full FP64 accuracy) */
+MBM_DEV double rcp(double x)
+{
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(e, r, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(e, r, r);
+}
+
+/* primitive variables {rho, v1, v2, theta, T11, T12, T22, q1, q2} */
+MBM_DEV void primitive(const double *U, double *P)
+{
+  const double ir = rcp(U[0]);
+  const double v1 = U[1] * ir, v2 = U[2] * ir;
+  const double vs = v1 * v1 + v2 * v2;
+  const double t11 = U[4] * ir - v1 * v1;
+  const double t12 = U[5] * ir - v1 * v2;
+  const double t22 = U[6] * ir - v2 * v2;
+  P[0] = U[0];
+  P[1] = v1;
+  P[2] = v2;
+  P[3] = (2.0 * U[3] * ir - vs) * (1.0 / 3.0);
+  P[4] = t11;
+  P[5] = t12;
+  P[6] = t22;
+  P[7] = U[7] - U[3] * v1 - U[0] * (t11 * v1 + t12 * v2);
+  P[8] = U[8] - U[3] * v2 - U[0] * (t12 * v1 + t22 * v2);
+}
+
+/* conservative variables from primitives, E recomputed (primDef.h) */
+MBM_DEV void conservative(const double *P, double *U)
+{
+  const double rho = P[0], v1 = P[1], v2 = P[2];
+  const double E = rho * (0.5 * (v1 * v1 + v2 * v2) + 1.5 * P[3]);
+  U[0] = rho;
+  U[1] = rho * v1;
+  U[2] = rho * v2;
+  U[3] = E;
+  U[4] = rho * (P[4] + v1 * v1);
+  U[5] = rho * (P[5] + v1 * v2);
+  U[6] = rho * (P[6] + v2 * v2);
+  U[7] = P[7] + E * v1 + rho * (P[4] * v1 + P[5] * v2);
+  U[8] = P[8] + E * v2 + rho * (P[5] * v1 + P[6] * v2);
+}
+
+/* damping factors of the relaxation for z = nu*dt.
+ * flag 0/-1: (1+z/2-z^2/4)/(1+z/2)^2, flag 1: (1+z)/(1+z/2)^2,
+ * flag 3: 1/(1+z/2); the second factor uses 2z/3 (Flux.c:247-248). */
+struct Damp { double d1, d2; };
+
+/* returns i1 = 1/(1+z/2), i2 = 1/(1+z/3) with one reciprocal */
+MBM_DEV void damp_inv(double z, double &i1, double &i2)
+{
+  const double a = 1.0 + 0.5 * z, b = 1.0 + (1.0 / 3.0) * z;
+  const double r = rcp(a * b);
+  i1 = b * r;
+  i2 = a * r;
+}
+
+MBM_DEV Damp damp0(double z, double i1, double i2)
+{
+  const double z3 = (2.0 / 3.0) * z;
+  Damp d;
+  d.d1 = (1.0 + 0.5 * z - 0.25 * z * z) * (i1 * i1);
+  d.d2 = (1.0 + 0.5 * z3 - 0.25 * z3 * z3) * (i2 * i2);
+  return d;
+}
+
+MBM_DEV Damp damp1(double z, double i1, double i2)
+{
+  Damp d;
+  d.d1 = (1.0 + z) * (i1 * i1);
+  d.d2 = (1.0 + (2.0 / 3.0) * z) * (i2 * i2);
+  return d;
+}
+
+/* relaxed primitives (Flux.c:249-253): T_ij -> theta + d1 (T_ij - theta),
+ * T12 -> d1 T12, q -> d2 q */
+MBM_DEV void relax(const double *P, Damp d, double *R)
+{
+  R[0] = P[0];
+  R[1] = P[1];
+  R[2] = P[2];
+  R[3] = P[3];
+  R[4] = __builtin_fma(d.d1, P[4] - P[3], P[3]);
+  R[5] = d.d1 * P[5];
+  R[6] = __builtin_fma(d.d1, P[6] - P[3], P[3]);
+  R[7] = d.d2 * P[7];
+  R[8] = d.d2 * P[8];
+}
+
+/* ---------------- slope limiters (Limiter.c:26-59) ---------------- */
+template <int LIM>
+MBM_DEV double limit(double d1, double d2)
+{
+  if constexpr (LIM == 0) {
+    return 0.0;
+  } else if constexpr (LIM == 1) {
+    return 0.5 * (d1 + d2);
+  } else if constexpr (LIM == 2) {
+    const double m = fmin(fabs(d1), fabs(d2));
+    return (d1 > 0.0 && d2 > 0.0) ? m : ((d1 < 0.0 && d2 < 0.0) ? -m : 0.0);
+  } else if constexpr (LIM == 3) {
+    /* d1 * phi(d2/d1), phi = max(0, min(2r,1), min(r,2)) */
+    const double a1 = fabs(d1), a2 = fabs(d2);
+    const double m = fmax(fmin(2.0 * a2, a1), fmin(a2, 2.0 * a1));
+    return (d1 > 0.0 && d2 > 0.0) ? m : ((d1 < 0.0 && d2 < 0.0) ? -m : 0.0);
+  } else {
+    const double w1 = fabs(d1) + 1.0e-10, w2 = fabs(d2) + 1.0e-10;
+    return (w1 * d2 + w2 * d1) * rcp(w1 + w2);
+  }
+}
+
+/* ---------------- Grad-13 flux (Flux.c:79-119) ----------------
+ * W: primitives in the face frame {rho, un, ut, theta, Tnn, Tnt, Ttt, qn, qt}.
+ * Side holds the products shared by the 9 components of the conservative
+ * state U and the normal flux F, so the components can be produced one at a
+ * time (keeps the register footprint of a face small). */
+struct Side {
+  double un2, ut2, vs, unut, E, An, At;
+};
+
+MBM_DEV Side side_of(const double *W)
+{
+  Side s;
+  s.un2 = W[1] * W[1];
+  s.ut2 = W[2] * W[2];
+  s.vs = s.un2 + s.ut2;
+  s.unut = W[1] * W[2];
+  s.E = W[0] * (0.5 * s.vs + 1.5 * W[3]);
+  s.An = W[4] * W[1] + W[5] * W[2];     /* (T.v)_n */
+  s.At = W[5] * W[1] + W[6] * W[2];     /* (T.v)_t */
+  return s;
+}
+
+/* component n of U and of the normal flux F */
+template <int N>
+MBM_DEV void uf(const double *W, const Side &s, double &u, double &f)
+{
+  const double rho = W[0], un = W[1], ut = W[2], th = W[3];
+  const double tnn = W[4], tnt = W[5], ttt = W[6], qn = W[7], qt = W[8];
+  if constexpr (N == 0) { u = rho; f = rho * un; }
+  else if constexpr (N == 1) { u = rho * un; f = rho * (tnn + s.un2); }
+  else if constexpr (N == 2) { u = rho * ut; f = rho * (tnt + s.unut); }
+  else if constexpr (N == 3) { u = s.E; f = qn + s.E * un + rho * s.An; }
+  else if constexpr (N == 4) { u = rho * (tnn + s.un2); f = 1.2 * qn + rho * un * (3.0 * tnn + s.un2); }
+  else if constexpr (N == 5) { u = rho * (tnt + s.unut); f = 0.4 * qt + rho * (tnn * ut + 2.0 * tnt * un + s.un2 * ut); }
+  else if constexpr (N == 6) { u = rho * (ttt + s.ut2); f = 0.4 * qn + rho * (ttt * un + 2.0 * tnt * ut + s.ut2 * un); }
+  else if constexpr (N == 7) {
+    u = qn + s.E * un + rho * s.An;
+    /* R11 = 5 th^2 + 7 th (Tnn - th) */
+    const double r11 = th * (5.0 * th + 7.0 * (tnn - th));
+    f = 2.0 * (qn + rho * s.An) * un + s.E * s.un2 + (1.2 * qn * un + 0.4 * qt * ut)
+      + 0.5 * rho * (tnn * s.vs + r11);
+  } else {
+    u = qt + s.E * ut + rho * s.At;
+    /* R12 = 7 th Tnt */
+    f = qt * un + qn * ut + rho * (s.unut * (tnn + ttt) + 1.5 * tnt * s.vs) + s.E * s.unut
+      + 0.4 * (qt * un + qn * ut) + 3.5 * rho * th * tnt;
+  }
+}
+
+/* full U and F (used where all components are needed at once) */
+MBM_DEV void cons_flux(const double *W, double *U, double *F)
+{
+  const Side s = side_of(W);
+  uf<0>(W, s, U[0], F[0]); uf<1>(W, s, U[1], F[1]); uf<2>(W, s, U[2], F[2]);
+  uf<3>(W, s, U[3], F[3]); uf<4>(W, s, U[4], F[4]); uf<5>(W, s, U[5], F[5]);
+  uf<6>(W, s, U[6], F[6]); uf<7>(W, s, U[7], F[7]); uf<8>(W, s, U[8], F[8]);
+}
+
+template <int N>
+MBM_DEV void hll_comp(const double *WL, const Side &sL, const double *WR, const Side &sR,
+                      double ca, double cb, double cc, double *Fs)
+{
+  double uL, fL, uR, fR;
+  uf<N>(WL, sL, uL, fL);
+  uf<N>(WR, sR, uR, fR);
+  Fs[N] = ca * fL + cb * fR + cc * (uR - uL);
+}
+
+/* HLL face flux (Flux.c:332-379), frame-local. Fs = (dt/dx) * flux.
+ * smax = |(un_L+un_R)/2| + max(c_L, c_R), c = 2 sqrt(5 theta / 3). */
+MBM_DEV void hll(const double *WL, const double *WR, double dtdx, double *Fs,
+                 double &smax)
+{
+  const double cL = sqrt((20.0 / 3.0) * WL[3]);
+  const double cR = sqrt((20.0 / 3.0) * WR[3]);
+  smax = fabs(0.5 * (WL[1] + WR[1])) + (cR > cL ? cR : cL);   /* max(cR,cL), tools.h:3 */
+  const double lamL = fmin(WL[1] - cL, 0.0);
+  const double lamR = fmax(WR[1] + cR, 0.0);
+  const double alph = lamR - lamL;
+  double ca, cb, cc;
+  if (alph > 0.0) {
+    const double s = dtdx * rcp(alph);
+    ca = lamR * s;
+    cb = -lamL * s;
+    cc = lamL * lamR * s;
+  } else {
+    ca = 0.5 * dtdx;
+    cb = 0.5 * dtdx;
+    cc = 0.0;
+  }
+  const Side sL = side_of(WL), sR = side_of(WR);
+  hll_comp<0>(WL, sL, WR, sR, ca, cb, cc, Fs);
+  hll_comp<1>(WL, sL, WR, sR, ca, cb, cc, Fs);
+  hll_comp<2>(WL, sL, WR, sR, ca, cb, cc, Fs);
+  hll_comp<3>(WL, sL, WR, sR, ca, cb, cc, Fs);
+  hll_comp<4>(WL, sL, WR, sR, ca, cb, cc, Fs);
+  hll_comp<5>(WL, sL, WR, sR, ca, cb, cc, Fs);
+  hll_comp<6>(WL, sL, WR, sR, ca, cb, cc, Fs);
+  hll_comp<7>(WL, sL, WR, sR, ca, cb, cc, Fs);
+  hll_comp<8>(WL, sL, WR, sR, ca, cb, cc, Fs);
+}
+
+/* x-face: frame = Cartesian. y-face: swap (1,2), (4,6), (7,8) in and out
+ * (the rotation of Flux.c:126-172 with n=(0,1) is exactly this permutation). */
+template <int DIR>
+MBM_DEV constexpr int perm(int n)
+{
+  if constexpr (DIR == 0) {
+    return n;
+  } else {
+    return n == 1 ? 2 : n == 2 ? 1 : n == 4 ? 6 : n == 6 ? 4 : n == 7 ? 8 : n == 8 ? 7 : n;
+  }
+}
+
+template <int DIR>
+MBM_DEV void face(const double *L, const double *R, double dtdx, double *Fs,
+                  double &smax)
+{
+  double WL[9], WR[9], Ff[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) {
+    WL[n] = L[perm<DIR>(n)];
+    WR[n] = R[perm<DIR>(n)];
+  }
+  hll(WL, WR, dtdx, Ff, smax);
+#pragma unroll
+  for (int n = 0; n < 9; n++) Fs[perm<DIR>(n)] = Ff[n];
+}
+
+} // namespace mbm
+
+#endif

@@ -1,0 +1,148 @@
+"""Parity of the gfx950 engine with the reference path (through the C ABI).
+
+Bar: FP64 relative L-inf per field <= 1e-12 (REL_LINF_TOL) against the FP64
+fixtures written by the compiled reference and against the CPU oracle; the
+step controller must take the same accept/reject decisions; integer/index
+work (ghost map, initial state) must be bit-exact.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import (GOLDEN, REL_LINF_TOL, deck_path, fixture_names, interior, load_fixture,
+                      rel_linf, scenario_of, split_state)
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def engine_for(mbm, name):
+    deck = mbm.read_deck(deck_path(name))
+    eng = mbm.Engine.from_deck(deck, scenario=scenario_of(name))
+    return deck, eng
+
+
+def oracle_for(deck, name):
+    return oracle.Oracle(deck.blocks, kn=deck.kn, force=deck.force, limiter=deck.limiter,
+                         scenario=scenario_of(name))
+
+
+@pytest.mark.parametrize("name", fixture_names())
+def test_initial_state_bitwise(mbm, name):
+    deck, eng = engine_for(mbm, name)
+    eng.init()
+    orc = oracle_for(deck, name)
+    orc.init()
+    for k, (g, r) in enumerate(zip(eng.download(0), orc.get(0))):
+        assert g.tobytes() == r.tobytes(), f"block {k + 1}"
+
+
+@pytest.mark.parametrize("name", fixture_names())
+def test_ghost_fill_matches_host_map_and_oracle(mbm, name):
+    deck, eng = engine_for(mbm, name)
+    rng = np.random.default_rng(7)
+    arrays = [rng.uniform(0.5, 2.0, eng.shape(k)) for k in range(deck.nblocks)]
+    eng.upload(arrays, layer=1)
+    eng.set_boundary(1, 0.0)
+    got = eng.download(1)
+    host = [a.copy() for a in arrays]
+    mbm.host_set_boundary(deck.blocks, host, kn=deck.kn, limiter=deck.limiter,
+                          scenario=scenario_of(name))
+    orc = oracle_for(deck, name)
+    orc.set([a.copy() for a in arrays], layer=1)
+    orc.set_boundary(1, 0.0)
+    for k in range(deck.nblocks):
+        assert got[k].tobytes() == host[k].tobytes(), f"GPU vs host map, block {k + 1}"
+        assert got[k].tobytes() == orc.get(1)[k].tobytes(), f"GPU vs oracle, block {k + 1}"
+
+
+@pytest.mark.parametrize("name", fixture_names())
+def test_one_time_update_matches_oracle(mbm, name):
+    deck, eng = engine_for(mbm, name)
+    eng.init()
+    orc = oracle_for(deck, name)
+    orc.init()
+    dt = 1e-4
+    c_gpu = eng.time_update(dt, 0.0)
+    c_ref = orc.time_update(dt, 0.0)
+    for layer in (1, 2):
+        g, r = eng.download(layer), orc.get(layer)
+        if name == "small_average":
+            for a, b in zip(g, r):
+                assert np.array_equal(np.isnan(a), np.isnan(b))
+            g = [np.nan_to_num(a) for a in g]
+            r = [np.nan_to_num(b) for b in r]
+        err = rel_linf(g, r)
+        assert err.max() <= REL_LINF_TOL, (layer, err)
+    assert abs(c_gpu - c_ref) <= REL_LINF_TOL * abs(c_ref), (c_gpu, c_ref)
+
+
+@pytest.mark.parametrize("name", fixture_names())
+def test_steps_match_reference_fixture(mbm, name):
+    state, ref_log = load_fixture(name)
+    nsteps = int(ref_log[:, 3].sum())
+    deck, eng = engine_for(mbm, name)
+    eng.init()
+    t, n, nrej, log = eng.steps(0.0, deck.tout[0], deck.cfl, maxsteps=nsteps)
+    assert n == nsteps
+    assert log.shape == ref_log.shape
+    assert np.array_equal(log[:, 3], ref_log[:, 3]), "accept/reject sequence differs"
+    assert np.allclose(log[:, :3], ref_log[:, :3], rtol=REL_LINF_TOL, atol=0.0, equal_nan=True)
+    ref = split_state(state, deck.blocks)
+    got = eng.download(0)
+    if name == "small_average":
+        for a, b in zip(got, ref):
+            assert np.array_equal(np.isnan(a), np.isnan(b))
+        got = [np.nan_to_num(a) for a in got]
+        ref = [np.nan_to_num(b) for b in ref]
+    err = rel_linf(got, ref)
+    assert err.max() <= REL_LINF_TOL, err
+
+
+def test_output_conversion_matches_oracle(mbm):
+    deck, eng = engine_for(mbm, "small_harmonic")
+    eng.init()
+    orc = oracle_for(deck, "small_harmonic")
+    orc.init()
+    eng.steps(0.0, deck.tout[0], deck.cfl, maxsteps=5)
+    orc.steps(deck.tout[0], deck.cfl, maxsteps=5)
+    for k in range(deck.nblocks):
+        g = eng.output_values(k)
+        r = orc.output_values(k)
+        scale = np.abs(r).max(axis=0)
+        assert (np.abs(g - r).max(axis=0) <= REL_LINF_TOL * np.where(scale > 0, scale, 1)).all()
+
+
+def run_moments(args, cwd):
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "build", "moments")
+    return subprocess.run([os.path.abspath(exe)] + args, cwd=cwd, capture_output=True,
+                          text=True, timeout=600)
+
+
+@pytest.mark.parametrize("name", ["squareSmall", "square4"])
+def test_moments_cli_against_golden(mbm, name):
+    """Drop-in run: same deck, same output files, compared like `make check`."""
+    from multiblockmomentsfv_amd.compare_solution import compare
+    import lzma
+    with tempfile.TemporaryDirectory() as tmp:
+        r = run_moments(["-bfile", deck_path(name), "-out", tmp], tmp)
+        assert r.returncode == 0, r.stdout + r.stderr
+        nst = 400 if name == "squareSmall" else 22500
+        base = os.path.join(tmp, f"Grad13Riemann2D_nst{nst}_Kn0p001_tAll")
+        ref = os.path.join(tmp, "ref")
+        with open(ref + ".hdr", "w") as f:
+            f.write(open(os.path.join(GOLDEN, name + ".ref.hdr")).read())
+        p = os.path.join(GOLDEN, name + ".ref.dat")
+        data = open(p, "rb").read() if os.path.exists(p) else lzma.decompress(open(p + ".xz", "rb").read())
+        with open(ref + ".dat", "wb") as f:
+            f.write(data)
+        msgs = []
+        assert compare(base, ref, 1e-6, 1e-5, out=msgs.append) == 0, msgs
+        mine = [l for l in open(base + ".hdr") if l.startswith("ZONE")]
+        gold = [l for l in open(ref + ".hdr") if l.startswith("ZONE")]
+        assert mine == gold
+        assert f"Done after {352 if name == 'square4' else 112} steps." in r.stdout
