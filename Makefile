@@ -22,7 +22,7 @@ LIB     := $(PKG)/libmbmfv.so
 BIN     := build/moments
 OBJDIR  := $(PKG)/_obj
 
-HIPSRC  := $(PKG)/csrc/mbm_engine.hip
+HIPSRC  := $(PKG)/csrc/mbm_engine.hip $(PKG)/csrc/mbm_adjoint.hip
 HIPHDR  := $(wildcard $(PKG)/csrc/*.h) include/mbmfv.h include/mbmfv_host.h
 
 .PHONY: all oracle clean
@@ -31,13 +31,13 @@ all: $(LIB) $(BIN)
 $(OBJDIR):
 	mkdir -p $(OBJDIR)
 
-$(OBJDIR)/mbm_engine.o: $(HIPSRC) $(HIPHDR) | $(OBJDIR)
+$(OBJDIR)/%.o: $(PKG)/csrc/%.hip $(HIPHDR) | $(OBJDIR)
 	$(HIPCC) $(HIPFLAGS) -c $< -o $@
 
 $(OBJDIR)/mbm_host.o: $(PKG)/host/mbm_host.c $(HIPHDR) | $(OBJDIR)
 	$(CC) $(CFLAGS) -c $< -o $@
 
-$(LIB): $(OBJDIR)/mbm_engine.o $(OBJDIR)/mbm_host.o
+$(LIB): $(OBJDIR)/mbm_engine.o $(OBJDIR)/mbm_adjoint.o $(OBJDIR)/mbm_host.o
 	$(HIPCC) --offload-arch=$(ARCH) -shared -o $@ $^
 
 $(BIN): $(PKG)/host/moments.c $(LIB) include/mbmfv.h include/mbmfv_host.h

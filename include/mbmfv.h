@@ -135,6 +135,35 @@ int mbm_enable_timing(mbm_ctx *c, int on);
 int mbm_kernel_time(mbm_ctx *c, double *stage_ms, int *nstage, double *fill_ms,
                     int *nfill);
 
+/* Whole-layer copies in the device layout above (all blocks, layer_elems
+ * doubles); src/dst may be host or device pointers. */
+long long mbm_layer_elems(const mbm_ctx *c);
+int mbm_layer_get(mbm_ctx *c, int layer, void *dst);
+int mbm_layer_set(mbm_ctx *c, int layer, const void *src);
+
+/* ---- discrete adjoint of the Heun step --------------------------------
+ * No counterpart in the reference (it has no derivatives). Reverse-mode
+ * derivative of a run of TimeUpdate steps with the dt schedule held fixed,
+ * with respect to the state and Kn; limiter, |.| and HLL speed clipping are
+ * differentiated almost everywhere. Memory: sqrt(N) checkpointing.
+ *
+ * mbm_adj_forward: advances DATA from its current value. dts == NULL: the
+ * Steps controller of Main.c:189-244 (from DT_INITIAL, rejection, clamp to
+ * tend, at most maxsteps accepted steps) and its accepted dt are recorded;
+ * otherwise the ndt given steps are replayed. DATA is checkpointed every
+ * 'stride' steps. */
+int mbm_adj_forward(mbm_ctx *c, double t0, double tend, double cflmax, double cflreg,
+                    int maxsteps, const double *dts, int ndt, int stride,
+                    int *nsteps, double *t_end);
+/* the dt schedule of the last mbm_adj_forward */
+int mbm_adj_schedule(mbm_ctx *c, double *dts, int cap, int *n);
+/* g_final: dJ/d(final DATA), layer layout (ghost entries are folded through
+ * the last ghost fill). g_initial receives dJ/d(initial DATA) on interior
+ * cells (ghost entries 0: the initial ghosts are a function of the interior
+ * through SetBoundary); *g_kn receives dJ/dKn. Host or device pointers. */
+int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial, double *g_kn);
+int mbm_adj_release(mbm_ctx *c);
+
 #ifdef __cplusplus
 }
 #endif

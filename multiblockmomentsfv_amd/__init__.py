@@ -107,6 +107,13 @@ def lib():
         "mbm_enable_timing": (i, [p, i]),
         "mbm_kernel_time": (i, [p, ctypes.POINTER(d), ctypes.POINTER(i), ctypes.POINTER(d),
                                 ctypes.POINTER(i)]),
+        "mbm_layer_elems": (ctypes.c_longlong, [p]),
+        "mbm_layer_get": (i, [p, i, p]),
+        "mbm_layer_set": (i, [p, i, p]),
+        "mbm_adj_forward": (i, [p, d, d, d, d, i, p, i, i, ctypes.POINTER(i), ctypes.POINTER(d)]),
+        "mbm_adj_schedule": (i, [p, p, i, ctypes.POINTER(i)]),
+        "mbm_adj_backward": (i, [p, p, p, ctypes.POINTER(d)]),
+        "mbm_adj_release": (i, [p]),
         "mbm_host_set_boundary": (i, [ctypes.POINTER(BlockDesc), i, ctypes.POINTER(Params),
                                       ctypes.POINTER(p), d]),
         "mbm_host_last_error": (ctypes.c_char_p, []),
@@ -139,7 +146,9 @@ def exported_symbols():
                         "mbm_download", "mbm_layer_ptr", "mbm_set_boundary",
                         "mbm_time_update", "mbm_commit", "mbm_steps", "mbm_output_values",
                         "mbm_interior_cells", "mbm_block_dims", "mbm_enable_timing",
-                        "mbm_kernel_time")]
+                        "mbm_kernel_time", "mbm_layer_elems", "mbm_layer_get", "mbm_layer_set",
+                        "mbm_adj_forward", "mbm_adj_schedule", "mbm_adj_backward",
+                        "mbm_adj_release")]
 
 
 def _check(rc, what):
@@ -316,6 +325,81 @@ class Engine:
 
     def enable_timing(self, on=True):
         _check(lib().mbm_enable_timing(self.h, int(on)), "mbm_enable_timing")
+
+    # ---- whole-layer access (device layout) and the discrete adjoint ----
+    def layer_elems(self):
+        return lib().mbm_layer_elems(self.h)
+
+    def layer_get(self, layer, out=None):
+        """Copies a whole layer (device layout) to `out` (numpy array or a
+        pointer-like object with .data_ptr(), e.g. a torch tensor)."""
+        if out is None:
+            out = np.empty(self.layer_elems())
+        ptr = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
+        _check(lib().mbm_layer_get(self.h, layer, ctypes.c_void_p(ptr)), "mbm_layer_get")
+        return out
+
+    def layer_set(self, layer, src):
+        ptr = src.data_ptr() if hasattr(src, "data_ptr") else np.ascontiguousarray(src).ctypes.data
+        _check(lib().mbm_layer_set(self.h, layer, ctypes.c_void_p(ptr)), "mbm_layer_set")
+
+    def set_kn(self, kn):
+        self.params.kn = float(kn)
+        _check(lib().mbm_set_params(self.h, ctypes.byref(self.params)), "mbm_set_params")
+
+    def set_stream(self, stream_handle):
+        _check(lib().mbm_set_stream(self.h, ctypes.c_void_p(stream_handle) if stream_handle else None),
+               "mbm_set_stream")
+
+    def adj_forward(self, tend, cflreg, cflmax=1.0, maxsteps=0, dts=None, stride=None, t0=0.0):
+        """Advances DATA recording (dts=None) or replaying a dt schedule,
+        with checkpoints every `stride` steps. Returns (nsteps, t_end)."""
+        n, te = ctypes.c_int(), ctypes.c_double()
+        if dts is not None:
+            dts = np.ascontiguousarray(dts, dtype=np.float64)
+            stride = stride or max(1, int(np.ceil(np.sqrt(len(dts)))))
+            _check(lib().mbm_adj_forward(self.h, t0, tend, cflmax, cflreg, 0, _ptr(dts), len(dts),
+                                         stride, ctypes.byref(n), ctypes.byref(te)), "mbm_adj_forward")
+        else:
+            stride = stride or max(1, int(np.ceil(np.sqrt(maxsteps if maxsteps > 0 else 1000))))
+            _check(lib().mbm_adj_forward(self.h, t0, tend, cflmax, cflreg, maxsteps, None, 0, stride,
+                                         ctypes.byref(n), ctypes.byref(te)), "mbm_adj_forward")
+        return n.value, te.value
+
+    def adj_schedule(self):
+        n = ctypes.c_int()
+        _check(lib().mbm_adj_schedule(self.h, None, 0, ctypes.byref(n)), "mbm_adj_schedule")
+        out = np.empty(n.value)
+        _check(lib().mbm_adj_schedule(self.h, _ptr(out), n.value, ctypes.byref(n)), "mbm_adj_schedule")
+        return out
+
+    def adj_backward(self, g_final, g_initial=None):
+        """dJ/d(initial DATA) and dJ/dKn from dJ/d(final DATA) (layer layout;
+        numpy arrays or torch tensors)."""
+        if g_initial is None:
+            g_initial = np.empty(self.layer_elems())
+        gp = g_final.data_ptr() if hasattr(g_final, "data_ptr") else np.ascontiguousarray(g_final).ctypes.data
+        ip = g_initial.data_ptr() if hasattr(g_initial, "data_ptr") else g_initial.ctypes.data
+        gk = ctypes.c_double()
+        _check(lib().mbm_adj_backward(self.h, ctypes.c_void_p(gp), ctypes.c_void_p(ip),
+                                      ctypes.byref(gk)), "mbm_adj_backward")
+        return g_initial, gk.value
+
+    def adj_release(self):
+        _check(lib().mbm_adj_release(self.h), "mbm_adj_release")
+
+    def layer_index(self):
+        """Per block (offset, plane, pitch) of the device layout."""
+        out = []
+        base = None
+        for k in range(len(self.blocks)):
+            dp, pl, pi = ctypes.c_void_p(), ctypes.c_longlong(), ctypes.c_int()
+            _check(lib().mbm_layer_ptr(self.h, 0, k, ctypes.byref(dp), ctypes.byref(pl), ctypes.byref(pi)),
+                   "mbm_layer_ptr")
+            if base is None:
+                base = dp.value
+            out.append(((dp.value - base) // 8, pl.value, pi.value))
+        return out
 
     def kernel_time(self):
         sm, fm = ctypes.c_double(), ctypes.c_double()

@@ -1,0 +1,143 @@
+"""Differentiable Heun runs: the engine's discrete adjoint behind
+torch.autograd.Function.
+
+The state is a float64 CUDA tensor in the engine's device layout (one whole
+layer, all blocks; ``Engine.layer_index`` gives each block's offset, plane
+stride and row pitch).  ``HeunRun.apply(U0, kn, engine, dts, stride)`` runs
+the steps of the frozen dt schedule ``dts`` from U0 (its ghost cells are
+recomputed by the ghost fill) and returns the final state; its backward is
+the hand-written adjoint of the C ABI (mbm_adj_backward), giving dJ/dU0 on
+interior cells and dJ/dKn.
+"""
+import numpy as np
+import torch
+
+from . import L_DATA, NV, Engine, read_deck
+
+
+class HeunRun(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, U0, kn, engine, dts, stride):
+        engine.set_stream(torch.cuda.current_stream().cuda_stream)
+        engine.set_kn(float(kn))
+        engine.layer_set(L_DATA, U0.detach().contiguous())
+        engine.set_boundary(L_DATA, 0.0)
+        engine.adj_forward(tend=float("inf"), cflreg=1.0, dts=dts, stride=stride)
+        UN = torch.empty_like(U0)
+        engine.layer_get(L_DATA, UN)
+        ctx.engine = engine
+        return UN
+
+    @staticmethod
+    def backward(ctx, gUN):
+        gU0 = torch.empty_like(gUN)
+        _, gkn = ctx.engine.adj_backward(gUN.contiguous(), gU0)
+        return gU0, torch.tensor(gkn, dtype=torch.float64, device=gUN.device), None, None, None
+
+
+def block_views(U, engine):
+    """(9, ny+6, nx+6) views of each block inside a layer-layout tensor."""
+    out = []
+    for (off, plane, pitch), b in zip(engine.layer_index(), engine.blocks):
+        lx, ly = b["nx"] + 6, b["ny"] + 6
+        v = U[off:off + NV * plane].view(NV, plane)[:, :ly * pitch].view(NV, ly, pitch)[:, :, :lx]
+        out.append(v)
+    return out
+
+
+def theta(Ub):
+    """theta = (2E/rho - |v|^2)/3 of a (9, ...) block view."""
+    rho = Ub[0]
+    v1, v2 = Ub[1] / rho, Ub[2] / rho
+    return (2 * Ub[3] / rho - v1 * v1 - v2 * v2) / 3
+
+
+def state_from_theta(th, rho, extra=None):
+    """Conservative state with temperature theta. extra: None (at rest,
+    T_ij = theta delta_ij, q = 0) or fields (rho~, v1, v2, dT11, dT22, T12, q1, q2)
+    added to the state at rest."""
+    z = torch.zeros_like(th)
+    if extra is None:
+        p = rho * th
+        return torch.stack([rho + z, z, z, 1.5 * p, p, z, p, z, z])
+    dr, v1, v2, d11, d22, t12, q1, q2 = extra
+    r = rho + dr
+    t11, t22 = th + d11, th + d22
+    E = r * (0.5 * (v1 * v1 + v2 * v2) + 1.5 * th)
+    return torch.stack([r + z, r * v1, r * v2, E, r * (t11 + v1 * v1), r * (t12 + v1 * v2),
+                        r * (t22 + v2 * v2), q1 + E * v1 + r * (t11 * v1 + t12 * v2),
+                        q2 + E * v2 + r * (t12 * v1 + t22 * v2)])
+
+
+def smooth_fields(X, Y, rng, amplitude):
+    """Eight smooth random fields (sums of low Fourier modes) on cell centres."""
+    out = []
+    for scale in (0.1, 0.1, 0.1, 0.05, 0.05, 0.05, 0.02, 0.02):
+        f = np.zeros_like(X)
+        for _ in range(3):
+            kx, ky, ph = rng.uniform(0.5, 3.0), rng.uniform(0.5, 3.0), rng.uniform(0, 2 * np.pi)
+            f += np.sin(kx * X + ky * Y + ph)
+        out.append(amplitude * scale * f / 3)
+    return out
+
+
+class TemperatureInversion:
+    """J(theta0, Kn) = sum over interior cells of (theta(t_end) - theta_target)^2
+    for a deck started from a temperature field theta0 at uniform density."""
+
+    def __init__(self, deck_path, kn=None, rho=1.0, seed=0, device="cuda", generic=0.0):
+        """generic > 0 adds smooth fields of that relative amplitude to every
+        primitive of the base state, so that no slope is exactly zero (at a
+        state at rest the limiters sit on their kinks and finite differences
+        measure secants, not derivatives)."""
+        self.deck = read_deck(deck_path)
+        self.kn = self.deck.kn if kn is None else kn
+        self.rho = rho
+        self.engine = Engine.from_deck(self.deck, scenario="tempjump")
+        self.engine.set_kn(self.kn)
+        self.device = device
+        self.engine.init()
+        U = torch.empty(self.engine.layer_elems(), dtype=torch.float64, device=device)
+        self.engine.layer_get(L_DATA, U)
+        self.U_init = U
+        self.theta0 = [theta(v)[3:-3, 3:-3].clone() for v in block_views(U, self.engine)]
+        rng = np.random.default_rng(seed)
+        self.target = []
+        self.extra = []
+        for k, b in enumerate(self.deck.blocks):
+            bx, by, dx, dy = self.deck.geom(k)
+            x = bx + dx * np.arange(3, b["nx"] + 3)
+            y = by + dy * np.arange(3, b["ny"] + 3)
+            X, Y = np.meshgrid(x, y)
+            cx, cy = rng.uniform(-0.3, 0.3, size=2)
+            amp, wid = rng.uniform(0.5, 1.5), rng.uniform(0.2, 0.4)
+            t = 1.0 + amp * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * wid ** 2))
+            self.target.append(torch.tensor(t, dtype=torch.float64, device=device))
+            if generic > 0:
+                self.extra.append([torch.tensor(f, dtype=torch.float64, device=device)
+                                   for f in smooth_fields(X, Y, rng, generic)])
+            else:
+                self.extra.append(None)
+        self.dts = None
+
+    def record_schedule(self, tend, maxsteps=0):
+        """Runs the adaptive controller once from the base state and keeps its dt."""
+        self.engine.layer_set(L_DATA, self.initial_state(self.theta0))
+        self.engine.set_boundary(L_DATA, 0.0)
+        n, t = self.engine.adj_forward(tend=tend, cflreg=self.deck.cfl, maxsteps=maxsteps)
+        self.dts = self.engine.adj_schedule()
+        return n, t
+
+    def initial_state(self, theta0):
+        U = self.U_init.clone()
+        for v, th, ex in zip(block_views(U, self.engine), theta0, self.extra):
+            v[:, 3:-3, 3:-3] = state_from_theta(th, self.rho, ex)
+        return U
+
+    def objective(self, theta0, kn, stride=None):
+        U0 = self.initial_state(theta0)
+        UN = HeunRun.apply(U0, kn, self.engine, self.dts, stride)
+        J = 0.0
+        for v, tgt in zip(block_views(UN, self.engine), self.target):
+            J = J + ((theta(v)[3:-3, 3:-3] - tgt) ** 2).sum()
+        return J

@@ -28,8 +28,9 @@
 #include "mbm_phys.h"
 #include "mbm_scenario.h"
 
-namespace {
+#include "mbm_internal.h"
 
+namespace mbmi {
 thread_local char g_err[512];
 
 int set_err(int code, const char *fmt, ...)
@@ -40,63 +41,29 @@ int set_err(int code, const char *fmt, ...)
   va_end(ap);
   return code;
 }
+} // namespace mbmi
 
-#define HIPCHK(expr)                                                           \
-  do {                                                                         \
-    hipError_t e_ = (expr);                                                    \
-    if (e_ != hipSuccess)                                                      \
-      return set_err(MBM_E_HIP, "%s failed: %s (%s:%d)", #expr,              \
-                     hipGetErrorString(e_), __FILE__, __LINE__);               \
-  } while (0)
-
-constexpr int NV = 9;
-constexpr int MBC = 3;
-constexpr int STRIP = 60;          /* owned columns per wavefront */
-constexpr int WAVES_PER_WG = 4;
-
-struct BlockDev {
-  long long off;      /* element offset of block in a layer buffer */
-  long long plane;    /* elements between components */
-  int lx, ly, pitch, pad;
-  double dx, dy, bx, by;
-};
-
-struct TileDev {
-  int blk;
-  int a0;             /* column of lane 0 */
-  int r0, r1;         /* first/last iteration row */
-};
-
-/* one ghost cell: value = sign * source, source is a cell of the same layer
- * or the scenario state at (x,y) */
-struct GhostDev {
-  int dblk, dcell;    /* destination block, b*pitch+a */
-  int sblk, scell;    /* source block/cell, sblk < 0: scenario */
-  int sign;           /* bit m set: component m negated */
-  int pad;
-  double x, y;
-};
-
-struct StageArgs {
-  const double *in;   /* stage 1: DATA, stage 2: TMP */
-  const double *in0;  /* stage 2: DATA */
-  double *out;        /* stage 1: TMP, stage 2: NEW */
-  const BlockDev *blocks;
-  const TileDev *tiles;
-  unsigned long long *slots;   /* per block: max x-face speed, y-face speed, prod CFL */
-  int ntiles;
-  double dt, dtkn, force;
-};
-
-} // namespace
+using namespace mbmi;
 
 /* ------------------------------------------------------------------ */
 /* cross-lane helpers                                                  */
 /* ------------------------------------------------------------------ */
 namespace mbm {
 
-MBM_DEV double from_left(double v) { return __shfl_up(v, 1, 64); }
-MBM_DEV double from_right(double v) { return __shfl_down(v, 1, 64); }
+/* wavefront-wide DPP shifts on the two 32-bit halves of a double:
+ * wave_shr:1 (lane i <- lane i-1), wave_shl:1 (lane i <- lane i+1); the lane
+ * without a source keeps its own value (never used by the stencil). */
+template <int CTRL>
+MBM_DEV double dpp_shift(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)b, hi = (int)(b >> 32);
+  const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+  const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)h2 << 32) | (unsigned int)l2);
+}
+MBM_DEV double from_left(double v) { return dpp_shift<0x138>(v); }
+MBM_DEV double from_right(double v) { return dpp_shift<0x130>(v); }
 
 MBM_DEV double wave_max(double v)
 {
@@ -115,59 +82,6 @@ MBM_DEV void load9(const double *__restrict__ p, long long plane, double *U)
 {
 #pragma unroll
   for (int m = 0; m < NV; m++) U[m] = p[m * plane];
-}
-
-/* Projection of one loaded cell (Main.c:314-336):
- *   P  = primitives used for slopes and faces: relaxed with flag 1 (stage 1)
- *        or 3 (stage 2) on rings <= 2, unrelaxed on ring 3;
- *   I  = initial value of the new state: relaxation flag 0 (stage 1) or
- *        flag 2 of the stage input plus flag -1 of DATA (stage 2);
- *   cp = production CFL 0.5*(1-R0(2z/3)) (stage 1 only). */
-template <int STAGE, bool NEED_INIT>
-MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
-                     double dtkn, double force, double *P, double *I, double &cp)
-{
-  double Q[NV];
-  primitive(U, Q);
-  const double z = Q[0] * sqrt(Q[3]) * dtkn;
-  double i1, i2;
-  damp_inv(z, i1, i2);
-  if constexpr (STAGE == 1) {
-    relax(Q, damp1(z, i1, i2), P);
-    if constexpr (NEED_INIT) {
-      const Damp d0 = damp0(z, i1, i2);
-      double R[NV];
-      relax(Q, d0, R);
-      R[1] += dt * force;
-      conservative(R, I);
-      cp = 0.5 * (1.0 - d0.d2);
-    }
-  } else {
-    Damp d3;
-    d3.d1 = i1;
-    d3.d2 = i2;
-    relax(Q, d3, P);
-    if constexpr (NEED_INIT) {
-      double R[NV], Q0[NV], V[NV];
-#pragma unroll
-      for (int n = 0; n < NV; n++) R[n] = Q[n];
-      R[1] += dt * force;
-      conservative(R, I);
-      primitive(U0, Q0);
-      const double z0 = Q0[0] * sqrt(Q0[3]) * dtkn;
-      double j1, j2;
-      damp_inv(z0, j1, j2);
-      relax(Q0, damp0(z0, j1, j2), R);
-      conservative(R, V);
-#pragma unroll
-      for (int n = 0; n < NV; n++) I[n] += V[n];
-      cp = 0.0;
-    }
-  }
-  if (ring3) {
-#pragma unroll
-    for (int n = 0; n < NV; n++) P[n] = Q[n];
-  }
 }
 
 } // namespace mbm
@@ -412,36 +326,10 @@ __global__ void output_kernel(const double *layer, const BlockDev *blocks, int b
 /* ------------------------------------------------------------------ */
 /* host side                                                           */
 /* ------------------------------------------------------------------ */
-struct mbm_ctx {
-  int nb;
-  std::vector<mbm_block_desc> desc;
-  std::vector<BlockDev> hblk;
-  mbm_params par;
-  size_t layer_elems;
-  double *layer[3];           /* physical buffers */
-  int role[3];                /* role[L] = physical buffer holding layer L */
-  BlockDev *dblk;
-  TileDev *dtiles;
-  int ntiles;
-  int tile_rows;
-  GhostDev *dghost;
-  int nghost;
-  unsigned long long *dslots; /* 2 stages x nb x 3 */
-  double *dcfl;               /* combined CFL */
-  double *dtj;                /* scenario params */
-  double *hcfl;               /* pinned */
-  hipStream_t stream;
-  bool own_stream;
-  bool timing;
-  hipEvent_t ev[4];
-  double stage_ms, fill_ms;
-  int nstage, nfill;
-  int device;
-};
 
 extern "C" const char *mbm_last_error(void) { return g_err; }
 
-namespace {
+namespace mbmi {
 
 /* ghost cell resolution: returns the final value source of ghost (a,b) of
  * block k after both sweeps of SetBoundary (Boundary.c:145-193). */
@@ -650,14 +538,33 @@ int launch_stage(mbm_ctx *c, const StageArgs &A)
   return MBM_OK;
 }
 
-int do_fill(mbm_ctx *c, int layer)
+int fill_buffer(mbm_ctx *c, double *layer)
 {
   if (c->nghost == 0) return MBM_OK;
   const int nblk = (c->nghost + 255) / 256;
-  hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, c->layer[c->role[layer]],
-                     c->dblk, c->dghost, c->nghost, c->par.scenario, c->dtj);
+  hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, layer, c->dblk, c->dghost,
+                     c->nghost, c->par.scenario, c->dtj);
   HIPCHK(hipGetLastError());
   return MBM_OK;
+}
+
+int do_fill(mbm_ctx *c, int layer) { return fill_buffer(c, c->layer[c->role[layer]]); }
+
+int enqueue_stage(mbm_ctx *c, int stage, const double *in, const double *in0, double *out,
+                  double dt, unsigned long long *slots)
+{
+  StageArgs A{};
+  A.blocks = c->dblk;
+  A.tiles = c->dtiles;
+  A.ntiles = c->ntiles;
+  A.dt = dt;
+  A.dtkn = dt / c->par.kn;
+  A.force = c->par.force;
+  A.in = in;
+  A.in0 = in0;
+  A.out = out;
+  A.slots = slots;
+  return stage == 1 ? launch_stage<1>(c, A) : launch_stage<2>(c, A);
 }
 
 int upload_params(mbm_ctx *c)
@@ -667,7 +574,7 @@ int upload_params(mbm_ctx *c)
   return MBM_OK;
 }
 
-} // namespace
+} // namespace mbmi
 
 extern "C" int mbm_create(const mbm_block_desc *blocks, int nblocks, const mbm_params *p,
                           void *stream, mbm_ctx **out)
@@ -864,7 +771,7 @@ extern "C" int mbm_set_boundary(mbm_ctx *c, int layer, double t)
   return MBM_OK;
 }
 
-namespace {
+namespace mbmi {
 
 /* enqueues one TimeUpdate; the combined CFL lands in c->dcfl */
 int enqueue_time_update(mbm_ctx *c, double dt)
@@ -914,7 +821,7 @@ int read_cfl(mbm_ctx *c, double *cfl)
   return MBM_OK;
 }
 
-} // namespace
+} // namespace mbmi
 
 extern "C" int mbm_time_update(mbm_ctx *c, double dt, double t, double *cfl)
 {

@@ -1,0 +1,113 @@
+/*
+ * mbm_internal.h -- structures shared by the engine's translation units
+ * (stage/fill kernels in mbm_engine.hip, adjoint in mbm_adjoint.hip).
+ */
+#ifndef MBM_INTERNAL_H
+#define MBM_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdarg>
+#include <vector>
+
+#include "mbmfv.h"
+#include "mbmfv_host.h"
+
+struct mbm_adj_state;
+
+namespace mbmi {
+
+int set_err(int code, const char *fmt, ...);
+
+#define HIPCHK(expr)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return set_err(MBM_E_HIP, "%s failed: %s (%s:%d)", #expr,              \
+                     hipGetErrorString(e_), __FILE__, __LINE__);               \
+  } while (0)
+
+constexpr int NV = 9;
+constexpr int MBC = 3;
+constexpr int STRIP = 60;          /* owned columns per wavefront */
+constexpr int WAVES_PER_WG = 4;
+
+struct BlockDev {
+  long long off;      /* element offset of block in a layer buffer */
+  long long plane;    /* elements between components */
+  int lx, ly, pitch, pad;
+  double dx, dy, bx, by;
+};
+
+struct TileDev {
+  int blk;
+  int a0;             /* column of lane 0 */
+  int r0, r1;         /* first/last iteration row */
+};
+
+/* one ghost cell: value = sign * source, source is a cell of the same layer
+ * or the scenario state at (x,y) */
+struct GhostDev {
+  int dblk, dcell;    /* destination block, b*pitch+a */
+  int sblk, scell;    /* source block/cell, sblk < 0: scenario */
+  int sign;           /* bit m set: component m negated */
+  int pad;
+  double x, y;
+};
+
+struct StageArgs {
+  const double *in;   /* stage 1: DATA, stage 2: TMP */
+  const double *in0;  /* stage 2: DATA */
+  double *out;        /* stage 1: TMP, stage 2: NEW */
+  const BlockDev *blocks;
+  const TileDev *tiles;
+  unsigned long long *slots;   /* per block: max x-face speed, y-face speed, prod CFL */
+  int ntiles;
+  double dt, dtkn, force;
+};
+
+
+} // namespace mbmi
+
+using mbmi::BlockDev;
+using mbmi::TileDev;
+using mbmi::GhostDev;
+
+struct mbm_ctx {
+  int nb;
+  std::vector<mbm_block_desc> desc;
+  std::vector<BlockDev> hblk;
+  mbm_params par;
+  size_t layer_elems;
+  double *layer[3];           /* physical buffers */
+  int role[3];                /* role[L] = physical buffer holding layer L */
+  BlockDev *dblk;
+  TileDev *dtiles;
+  int ntiles;
+  int tile_rows;
+  GhostDev *dghost;
+  int nghost;
+  unsigned long long *dslots; /* 2 stages x nb x 3 */
+  double *dcfl;               /* combined CFL */
+  double *dtj;                /* scenario params */
+  double *hcfl;               /* pinned */
+  hipStream_t stream;
+  bool own_stream;
+  bool timing;
+  hipEvent_t ev[4];
+  double stage_ms, fill_ms;
+  int nstage, nfill;
+  int device;
+  mbm_adj_state *adj;         /* adjoint buffers, created on demand */
+};
+
+namespace mbmi {
+std::vector<GhostDev> build_ghost_map(const std::vector<mbm_block_desc> &D,
+                                      const std::vector<BlockDev> &hb);
+int do_fill(mbm_ctx *c, int layer);
+int fill_buffer(mbm_ctx *c, double *layer);
+int enqueue_stage(mbm_ctx *c, int stage, const double *in, const double *in0, double *out,
+                  double dt, unsigned long long *slots);
+} // namespace mbmi
+
+#endif

@@ -20,20 +20,31 @@
 #ifndef MBM_PHYS_H
 #define MBM_PHYS_H
 
-#include <hip/hip_runtime.h>
+#include <math.h>
 
-#define MBM_DEV __device__ __forceinline__
+/* The functions are host+device so the same code can be unit-tested on the
+ * CPU (the host build uses IEEE division for rcp). */
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MBM_DEV __host__ __device__ __forceinline__
+#else
+#define MBM_DEV static inline
+#endif
 
 namespace mbm {
 
 /* reciprocal: hardware estimate + two Newton steps (full FP64 accuracy) */
 MBM_DEV double rcp(double x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rcp(x);
   double e = __builtin_fma(-x, r, 1.0);
   r = __builtin_fma(e, r, r);
   e = __builtin_fma(-x, r, 1.0);
   return __builtin_fma(e, r, r);
+#else
+  return 1.0 / x;
+#endif
 }
 
 /* primitive variables {rho, v1, v2, theta, T11, T12, T22, q1, q2} */
@@ -267,6 +278,59 @@ MBM_DEV void face(const double *L, const double *R, double dtdx, double *Fs,
   hll(WL, WR, dtdx, Ff, smax);
 #pragma unroll
   for (int n = 0; n < 9; n++) Fs[perm<DIR>(n)] = Ff[n];
+}
+
+/* Projection of one loaded cell (Main.c:314-336):
+ *   P  = primitives used for slopes and faces: relaxed with flag 1 (stage 1)
+ *        or 3 (stage 2) on rings <= 2, unrelaxed on ring 3;
+ *   I  = initial value of the new state: relaxation flag 0 (stage 1) or
+ *        flag 2 of the stage input plus flag -1 of DATA (stage 2);
+ *   cp = production CFL 0.5*(1-R0(2z/3)) (stage 1 only). */
+template <int STAGE, bool NEED_INIT>
+MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
+                     double dtkn, double force, double *P, double *I, double &cp)
+{
+  double Q[9];
+  primitive(U, Q);
+  const double z = Q[0] * sqrt(Q[3]) * dtkn;
+  double i1, i2;
+  damp_inv(z, i1, i2);
+  if constexpr (STAGE == 1) {
+    relax(Q, damp1(z, i1, i2), P);
+    if constexpr (NEED_INIT) {
+      const Damp d0 = damp0(z, i1, i2);
+      double R[9];
+      relax(Q, d0, R);
+      R[1] += dt * force;
+      conservative(R, I);
+      cp = 0.5 * (1.0 - d0.d2);
+    }
+  } else {
+    Damp d3;
+    d3.d1 = i1;
+    d3.d2 = i2;
+    relax(Q, d3, P);
+    if constexpr (NEED_INIT) {
+      double R[9], Q0[9], V[9];
+#pragma unroll
+      for (int n = 0; n < 9; n++) R[n] = Q[n];
+      R[1] += dt * force;
+      conservative(R, I);
+      primitive(U0, Q0);
+      const double z0 = Q0[0] * sqrt(Q0[3]) * dtkn;
+      double j1, j2;
+      damp_inv(z0, j1, j2);
+      relax(Q0, damp0(z0, j1, j2), R);
+      conservative(R, V);
+#pragma unroll
+      for (int n = 0; n < 9; n++) I[n] += V[n];
+      cp = 0.0;
+    }
+  }
+  if (ring3) {
+#pragma unroll
+    for (int n = 0; n < 9; n++) P[n] = Q[n];
+  }
 }
 
 } // namespace mbm
