@@ -3,6 +3,8 @@
 #   make            multiblockmomentsfv_amd/libmbmfv.so (engine + host C) and
 #                   build/moments (drop-in for the reference's ./build/moments)
 #   make oracle     the CPU oracle used by the tests (oracle/liboracle.so)
+#   make physcheck  host build of the engine's per-cell functions and their
+#                   VJPs for the adjoint tests (tests/native/libphyscheck.so)
 #   make clean
 #
 # All HIP code is compiled for gfx950 only.
@@ -22,11 +24,19 @@ LIB     := $(PKG)/libmbmfv.so
 BIN     := build/moments
 OBJDIR  := $(PKG)/_obj
 
-HIPSRC  := $(PKG)/csrc/mbm_engine.hip $(PKG)/csrc/mbm_adjoint.hip
+HIPSRC  := $(PKG)/csrc/mbm_engine.hip $(PKG)/csrc/mbm_adjoint.hip $(PKG)/csrc/mbm_batch.hip
 HIPHDR  := $(wildcard $(PKG)/csrc/*.h) include/mbmfv.h include/mbmfv_host.h
 
-.PHONY: all oracle clean
+CXX     ?= g++
+PHYSCHK := tests/native/libphyscheck.so
+
+.PHONY: all oracle physcheck clean
 all: $(LIB) $(BIN)
+
+physcheck: $(PHYSCHK)
+
+$(PHYSCHK): tests/native/physcheck.cpp $(HIPHDR)
+	$(CXX) -std=c++17 -O2 -fPIC -shared -ffp-contract=off -Iinclude -o $@ $<
 
 $(OBJDIR):
 	mkdir -p $(OBJDIR)
@@ -37,7 +47,7 @@ $(OBJDIR)/%.o: $(PKG)/csrc/%.hip $(HIPHDR) | $(OBJDIR)
 $(OBJDIR)/mbm_host.o: $(PKG)/host/mbm_host.c $(HIPHDR) | $(OBJDIR)
 	$(CC) $(CFLAGS) -c $< -o $@
 
-$(LIB): $(OBJDIR)/mbm_engine.o $(OBJDIR)/mbm_adjoint.o $(OBJDIR)/mbm_host.o
+$(LIB): $(OBJDIR)/mbm_engine.o $(OBJDIR)/mbm_adjoint.o $(OBJDIR)/mbm_batch.o $(OBJDIR)/mbm_host.o
 	$(HIPCC) --offload-arch=$(ARCH) -shared -o $@ $^
 
 $(BIN): $(PKG)/host/moments.c $(LIB) include/mbmfv.h include/mbmfv_host.h
@@ -48,4 +58,4 @@ oracle:
 	$(MAKE) -C oracle
 
 clean:
-	rm -rf $(OBJDIR) $(LIB) build
+	rm -rf $(OBJDIR) $(LIB) build $(PHYSCHK)

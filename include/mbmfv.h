@@ -129,8 +129,10 @@ int mbm_output_values(mbm_ctx *c, int block, int float32, void *host);
 long long mbm_interior_cells(const mbm_ctx *c);
 int mbm_block_dims(const mbm_ctx *c, int block, int *nx, int *ny);
 
-/* Stage-kernel timing: mean device time of the last timed stage launches
- * (ms) recorded with HIP events when timing is enabled. */
+/* Kernel timing of mbm_time_update calls made while timing is enabled: HIP
+ * events on the context's stream bracket each stage launch and the TMP ghost
+ * fill; mbm_kernel_time returns the summed ms and the launch counts since
+ * mbm_enable_timing. (mbm_steps replays a captured graph and is not timed.) */
 int mbm_enable_timing(mbm_ctx *c, int on);
 int mbm_kernel_time(mbm_ctx *c, double *stage_ms, int *nstage, double *fill_ms,
                     int *nfill);
@@ -163,6 +165,27 @@ int mbm_adj_schedule(mbm_ctx *c, double *dts, int cap, int *n);
  * through SetBoundary); *g_kn receives dJ/dKn. Host or device pointers. */
 int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial, double *g_kn);
 int mbm_adj_release(mbm_ctx *c);
+
+/* ---- batches of independent decks ---------------------------------------
+ * No counterpart in the reference (one deck per process there). Decks are
+ * listed back to back in 'blocks'; each deck's sync codes are 1-based within
+ * the deck. Per deck: Kn, F (params[d]) and its own Steps controller (dt,
+ * rejection, tend); the limiter and scenario must be common. The returned
+ * context works with mbm_init, mbm_set_tile_rows, mbm_layer_* and
+ * mbm_destroy; mbm_time_update/mbm_steps are for single decks. */
+int mbm_batch_create(const mbm_block_desc *blocks, const int *nblocks_per_deck, int ndecks,
+                     const mbm_params *params, void *stream, mbm_ctx **out);
+int mbm_batch_ndecks(const mbm_ctx *c);
+/* Advances every deck d from t[d] (dt from DT_INITIAL) with its controller
+ * until t[d] >= tend[d] or maxsteps accepted steps; one hipGraph-captured
+ * step of all decks per launch sequence, no host round trip per step. */
+int mbm_batch_run(mbm_ctx *c, const double *tend, const double *cflreg, double cflmax,
+                  int maxsteps, double *t, int *nsteps, int *nrej);
+/* DATA of one block of one deck, AoS as mbm_download */
+int mbm_batch_download(mbm_ctx *c, int deck, int block, double *host);
+/* Times the candidate tile heights on the batch and keeps the fastest. */
+int mbm_batch_autotune(mbm_ctx *c, const int *rows, int nrows, int reps, int *best,
+                       double *best_ms);
 
 #ifdef __cplusplus
 }

@@ -114,6 +114,12 @@ def lib():
         "mbm_adj_schedule": (i, [p, p, i, ctypes.POINTER(i)]),
         "mbm_adj_backward": (i, [p, p, p, ctypes.POINTER(d)]),
         "mbm_adj_release": (i, [p]),
+        "mbm_batch_create": (i, [ctypes.POINTER(BlockDesc), p, i, ctypes.POINTER(Params), p,
+                                 ctypes.POINTER(p)]),
+        "mbm_batch_ndecks": (i, [p]),
+        "mbm_batch_run": (i, [p, p, p, d, i, p, p, p]),
+        "mbm_batch_download": (i, [p, i, i, p]),
+        "mbm_batch_autotune": (i, [p, p, i, i, ctypes.POINTER(i), ctypes.POINTER(d)]),
         "mbm_host_set_boundary": (i, [ctypes.POINTER(BlockDesc), i, ctypes.POINTER(Params),
                                       ctypes.POINTER(p), d]),
         "mbm_host_last_error": (ctypes.c_char_p, []),
@@ -148,7 +154,8 @@ def exported_symbols():
                         "mbm_interior_cells", "mbm_block_dims", "mbm_enable_timing",
                         "mbm_kernel_time", "mbm_layer_elems", "mbm_layer_get", "mbm_layer_set",
                         "mbm_adj_forward", "mbm_adj_schedule", "mbm_adj_backward",
-                        "mbm_adj_release")]
+                        "mbm_adj_release", "mbm_batch_create", "mbm_batch_ndecks",
+                        "mbm_batch_run", "mbm_batch_download", "mbm_batch_autotune")]
 
 
 def _check(rc, what):
@@ -253,8 +260,9 @@ class Engine:
 
     @classmethod
     def from_deck(cls, deck, scenario="riemann2d", **kw):
-        return cls(deck.blocks, kn=deck.kn, force=deck.force, limiter=deck.limiter,
-                   scenario=scenario, **kw)
+        args = dict(kn=deck.kn, force=deck.force, limiter=deck.limiter, scenario=scenario)
+        args.update(kw)
+        return cls(deck.blocks, **args)
 
     def close(self):
         if getattr(self, "h", None):
@@ -407,6 +415,106 @@ class Engine:
         _check(lib().mbm_kernel_time(self.h, ctypes.byref(sm), ctypes.byref(ns), ctypes.byref(fm),
                                      ctypes.byref(nf)), "mbm_kernel_time")
         return sm.value, ns.value, fm.value, nf.value
+
+
+def _desc_array(blocks):
+    arr = (BlockDesc * len(blocks))()
+    for k, b in enumerate(blocks):
+        arr[k].nx, arr[k].ny = b["nx"], b["ny"]
+        arr[k].xmin, arr[k].xmax, arr[k].ymin, arr[k].ymax = b["xmin"], b["xmax"], b["ymin"], b["ymax"]
+        for s in range(4):
+            arr[k].bc[s] = b["bc"][s]
+    return arr
+
+
+class Batch:
+    """Independent decks advanced together (mbm_batch_*): per-deck Kn and
+    Steps controller, shared limiter and scenario."""
+
+    def __init__(self, decks, limiter="harmonic", scenario="riemann2d",
+                 scenario_params=TEMPJUMP_DEFAULT, stream=None):
+        """decks: list of dicts with keys blocks (deck-local sync codes), kn,
+        and optionally force, cfl, tend."""
+        self.decks = decks
+        blocks = [b for d in decks for b in d["blocks"]]
+        self._desc = _desc_array(blocks)
+        self._nb = np.array([len(d["blocks"]) for d in decks], dtype=np.int32)
+        self._par = (Params * len(decks))()
+        for i, d in enumerate(decks):
+            self._par[i].kn = d["kn"]
+            self._par[i].force = d.get("force", 0.0)
+            self._par[i].limiter = LIMITERS[limiter]
+            self._par[i].scenario = SCENARIOS[scenario]
+            for j, v in enumerate(scenario_params):
+                self._par[i].scenario_params[j] = v
+        h = ctypes.c_void_p()
+        _check(lib().mbm_batch_create(self._desc, _ptr(self._nb), len(decks), self._par,
+                                      ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)),
+               "mbm_batch_create")
+        self.h = h
+        self.t = np.zeros(len(decks))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self):
+        _check(lib().mbm_init(self.h), "mbm_init")
+        self.t[:] = 0.0
+
+    def cells(self):
+        return sum(b["nx"] * b["ny"] for d in self.decks for b in d["blocks"])
+
+    def autotune(self, rows=(8, 16, 24, 32, 48, 64), reps=3):
+        r = np.array(rows, dtype=np.int32)
+        best, ms = ctypes.c_int(), ctypes.c_double()
+        _check(lib().mbm_batch_autotune(self.h, _ptr(r), len(r), reps, ctypes.byref(best),
+                                        ctypes.byref(ms)), "mbm_batch_autotune")
+        return best.value, ms.value
+
+    def run(self, maxsteps, tend=None, cflmax=1.0):
+        n = len(self.decks)
+        te = np.array([d.get("tend", 1e30) if tend is None else tend for d in self.decks], dtype=np.float64)
+        cr = np.array([d.get("cfl", 0.95) for d in self.decks], dtype=np.float64)
+        ns = np.zeros(n, dtype=np.int32)
+        nr = np.zeros(n, dtype=np.int32)
+        _check(lib().mbm_batch_run(self.h, _ptr(te), _ptr(cr), cflmax, maxsteps, _ptr(self.t),
+                                   _ptr(ns), _ptr(nr)), "mbm_batch_run")
+        return ns, nr
+
+    def download(self, deck):
+        out = []
+        for k, b in enumerate(self.decks[deck]["blocks"]):
+            a = np.empty((b["ny"] + 6, b["nx"] + 6, NV))
+            _check(lib().mbm_batch_download(self.h, deck, k, _ptr(a)), "mbm_batch_download")
+            out.append(a)
+        return out
+
+
+def square4_topology(a, b, c, d, x0=-1.0, xm=0.5, x1=1.0, y0=-1.0, ym=0.2, y1=1.0):
+    """cases/square4.txt geometry with nx(B1)=nx(B4)=a, nx(B2)=nx(B3)=b,
+    ny(B1)=ny(B2)=c, ny(B3)=ny(B4)=d (the sizes sync conformity allows)."""
+    return [dict(nx=a, ny=c, xmin=x0, xmax=xm, ymin=y0, ymax=ym, bc=[0, 2, -1, 4]),
+            dict(nx=b, ny=c, xmin=xm, xmax=x1, ymin=y0, ymax=ym, bc=[1, 0, -1, 3]),
+            dict(nx=b, ny=d, xmin=xm, xmax=x1, ymin=ym, ymax=y1, bc=[4, 0, 2, -1]),
+            dict(nx=a, ny=d, xmin=x0, xmax=xm, ymin=ym, ymax=y1, bc=[0, 3, 1, -1])]
+
+
+def sweep_decks(n=256, seed=0, sides=tuple(range(64, 513, 32)), kn_range=(1e-3, 1e-1), cfl=0.95):
+    """Seeded sweep of square4-topology decks: block sides drawn from
+    {64, 96, ..., 512}, Kn log-uniform, R13 in {0, 1} (output naming only)."""
+    rng = np.random.default_rng(seed)
+    decks = []
+    for _ in range(n):
+        a, b, c, d = (int(rng.choice(sides)) for _ in range(4))
+        kn = float(10 ** rng.uniform(np.log10(kn_range[0]), np.log10(kn_range[1])))
+        decks.append(dict(blocks=square4_topology(a, b, c, d), kn=kn, r13=int(rng.integers(0, 2)),
+                          cfl=cfl))
+    return decks
 
 
 def host_set_boundary(blocks, arrays, kn=0.001, limiter="harmonic", scenario="riemann2d",

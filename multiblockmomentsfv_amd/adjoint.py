@@ -85,7 +85,8 @@ class TemperatureInversion:
     """J(theta0, Kn) = sum over interior cells of (theta(t_end) - theta_target)^2
     for a deck started from a temperature field theta0 at uniform density."""
 
-    def __init__(self, deck_path, kn=None, rho=1.0, seed=0, device="cuda", generic=0.0):
+    def __init__(self, deck_path, kn=None, rho=1.0, seed=0, device="cuda", generic=0.0,
+                 limiter=None):
         """generic > 0 adds smooth fields of that relative amplitude to every
         primitive of the base state, so that no slope is exactly zero (at a
         state at rest the limiters sit on their kinks and finite differences
@@ -93,7 +94,8 @@ class TemperatureInversion:
         self.deck = read_deck(deck_path)
         self.kn = self.deck.kn if kn is None else kn
         self.rho = rho
-        self.engine = Engine.from_deck(self.deck, scenario="tempjump")
+        kw = {} if limiter is None else {"limiter": limiter}
+        self.engine = Engine.from_deck(self.deck, scenario="tempjump", **kw)
         self.engine.set_kn(self.kn)
         self.device = device
         self.engine.init()

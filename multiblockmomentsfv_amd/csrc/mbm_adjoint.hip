@@ -21,6 +21,7 @@
  */
 #include "mbm_internal.h"
 #include "mbm_adj.h"
+#include "mbm_adj_stage.h"
 
 #include <algorithm>
 #include <map>
@@ -46,27 +47,9 @@ struct mbm_adj_state {
 };
 
 namespace {
-
 constexpr int ADJ_THREADS = 256;
-
-MBM_DEV bool interior(int a, int b, int lx, int ly)
-{
-  return a >= MBC && a <= lx - 1 - MBC && b >= MBC && b <= ly - 1 - MBC;
 }
 
-MBM_DEV void ld(const double *base, long long plane, long long off, double *v)
-{
-  for (int m = 0; m < NV; m++) v[m] = base[off + m * plane];
-}
-
-MBM_DEV void st(double *base, long long plane, long long off, const double *v)
-{
-  for (int m = 0; m < NV; m++) base[off + m * plane] = v[m];
-}
-
-} // namespace
-
-/* primitives of the stage input (Main.c:314-336), every cell */
 template <int STAGE>
 __global__ void __launch_bounds__(ADJ_THREADS) adj_prim(const double *in, double *P,
                                                         const BlockDev *blocks, double dt,
@@ -75,18 +58,9 @@ __global__ void __launch_bounds__(ADJ_THREADS) adj_prim(const double *in, double
   const BlockDev B = blocks[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= B.lx * B.ly) return;
-  const int a = i % B.lx, b = i / B.lx;
-  const long long off = B.off + (long long)b * B.pitch + a;
-  double U[NV], Q[NV], I[NV];
-  double cp;
-  ld(in, B.plane, off, U);
-  const bool ring3 = a == 0 || a == B.lx - 1 || b == 0 || b == B.ly - 1;
-  project<STAGE, false>(U, U, ring3, dt, dtkn, force, Q, I, cp);
-  st(P, B.plane, off, Q);
+  adj_prim_cell<STAGE>(in, P, B, i % B.lx, i / B.lx, dt, dtkn, force);
 }
 
-/* face-state adjoints. x-face a+1/2 (row b) stored at cell (a,b) in FxL (left
- * state, from cell a) and FxR (right state, from cell a+1); y likewise. */
 template <int LIM>
 __global__ void __launch_bounds__(ADJ_THREADS) adj_face(const double *P, const double *gout,
                                                         double *FxL, double *FxR, double *FyL,
@@ -96,45 +70,9 @@ __global__ void __launch_bounds__(ADJ_THREADS) adj_face(const double *P, const d
   const BlockDev B = blocks[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= B.lx * B.ly) return;
-  const int lx = B.lx, ly = B.ly;
-  const int a = i % lx, b = i / lx;
-  const long long pl = B.plane;
-  const long long off = B.off + (long long)b * B.pitch + a;
-  double gL[NV], gR[NV];
-  for (int d = 0; d < 2; d++) {
-    for (int n = 0; n < NV; n++) { gL[n] = 0.0; gR[n] = 0.0; }
-    const bool in_set = d == 0 ? (a >= 2 && a <= lx - 4 && b >= MBC && b <= ly - 1 - MBC)
-                               : (b >= 2 && b <= ly - 4 && a >= MBC && a <= lx - 1 - MBC);
-    if (in_set) {
-      const long long step = d == 0 ? 1 : B.pitch;
-      double Pm[NV], P0[NV], P1[NV], P2[NV], WL[NV], WR[NV], gF[NV];
-      ld(P, pl, off - step, Pm);
-      ld(P, pl, off, P0);
-      ld(P, pl, off + step, P1);
-      ld(P, pl, off + 2 * step, P2);
-      for (int n = 0; n < NV; n++) {
-        const double s0 = limit<LIM>(P0[n] - Pm[n], P1[n] - P0[n]);
-        const double s1 = limit<LIM>(P1[n] - P0[n], P2[n] - P1[n]);
-        WL[n] = P0[n] + 0.5 * s0;
-        WR[n] = P1[n] - 0.5 * s1;
-      }
-      const int a1 = d == 0 ? a + 1 : a, b1 = d == 0 ? b : b + 1;
-      const double w0 = interior(a, b, lx, ly) ? scale : 0.0;
-      const double w1 = interior(a1, b1, lx, ly) ? scale : 0.0;
-      double g0[NV], g1[NV];
-      ld(gout, pl, off, g0);
-      ld(gout, pl, off + step, g1);
-      for (int n = 0; n < NV; n++) gF[n] = w1 * g1[n] - w0 * g0[n];
-      if (d == 0) face_vjp<0>(WL, WR, dt / B.dx, gF, gL, gR);
-      else face_vjp<1>(WL, WR, dt / B.dy, gF, gL, gR);
-    }
-    if (d == 0) { st(FxL, pl, off, gL); st(FxR, pl, off, gR); }
-    else { st(FyL, pl, off, gL); st(FyR, pl, off, gR); }
-  }
+  adj_face_cell<LIM>(P, gout, FxL, FxR, FyL, FyR, B, i % B.lx, i / B.lx, dt, scale);
 }
 
-/* per cell: gP from the faces and slopes, then the projection/init VJP.
- * stage 1: gD += S1^T; stage 2: gT = S2^T (stage input), gD += Data0 part. */
 template <int STAGE, int LIM>
 __global__ void __launch_bounds__(ADJ_THREADS) adj_cell(
     const double *in, const double *in0, const double *P, const double *FxL, const double *FxR,
@@ -144,110 +82,9 @@ __global__ void __launch_bounds__(ADJ_THREADS) adj_cell(
   const BlockDev B = blocks[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double gkn = 0.0;
-  if (i < B.lx * B.ly) {
-    const int lx = B.lx, ly = B.ly;
-    const int a = i % lx, b = i / lx;
-    const long long pl = B.plane;
-    const long long off = B.off + (long long)b * B.pitch + a;
-    double gU[NV] = { 0 }, gU0[NV] = { 0 };
-    if (a >= 1 && a <= lx - 2 && b >= 1 && b <= ly - 2) {
-      double gP[NV] = { 0 };
-      for (int d = 0; d < 2; d++) {
-        const long long step = d == 0 ? 1 : B.pitch;
-        const int pos = d == 0 ? a : b, len = d == 0 ? lx : ly;
-        const double *FL = d == 0 ? FxL : FyL;
-        const double *FR = d == 0 ? FxR : FyR;
-        double v[NV], w[NV];
-        /* direct: left state of face pos+1/2, right state of face pos-1/2 */
-        ld(FL, pl, off, v);
-        ld(FR, pl, off - step, w);
-        for (int n = 0; n < NV; n++) gP[n] += v[n] + w[n];
-        /* slopes at e = pos-1, pos, pos+1 use P(pos) */
-        for (int k = -1; k <= 1; k++) {
-          const int e = pos + k;
-          if (e < 2 || e > len - 3) continue;   /* slopes used by faces in the set */
-          const long long oe = off + k * step;
-          double fl[NV], fr[NV], pm[NV], p0[NV], pp[NV];
-          ld(FL, pl, oe, fl);
-          ld(FR, pl, oe - step, fr);
-          ld(P, pl, oe - step, pm);
-          ld(P, pl, oe, p0);
-          ld(P, pl, oe + step, pp);
-          for (int n = 0; n < NV; n++) {
-            const double gs = 0.5 * fl[n] - 0.5 * fr[n];
-            double gd1 = 0.0, gd2 = 0.0;
-            limit_vjp<LIM>(p0[n] - pm[n], pp[n] - p0[n], gs, gd1, gd2);
-            /* d1(e) = P(e) - P(e-1), d2(e) = P(e+1) - P(e) */
-            if (k == 0) gP[n] += gd1 - gd2;
-            else if (k == -1) gP[n] += gd2;
-            else gP[n] -= gd1;
-          }
-        }
-      }
-      double gI[NV], U[NV], Q[NV];
-      ld(gout, pl, off, gI);
-      const double wI = interior(a, b, lx, ly) ? scale : 0.0;
-      for (int n = 0; n < NV; n++) gI[n] *= wI;
-      ld(in, pl, off, U);
-      primitive(U, Q);
-      const double dtkn = dt / kn;
-      const double z = Q[0] * sqrt(Q[3]) * dtkn;
-      double i1, i2;
-      damp_inv(z, i1, i2);
-      double gQ[NV] = { 0 }, gz = 0.0, g1 = 0.0, g2 = 0.0;
-      if (STAGE == 1) {
-        relax_vjp(Q, damp1(z, i1, i2), gP, gQ, g1, g2);
-        Damp dz = damp1_dz(z, i1, i2);
-        gz += dz.d1 * g1 + dz.d2 * g2;
-        const Damp d0 = damp0(z, i1, i2);
-        double R[NV], gR[NV] = { 0 };
-        relax(Q, d0, R);
-        R[1] += dt * force;
-        conservative_vjp(R, gI, gR);
-        g1 = 0.0;
-        g2 = 0.0;
-        relax_vjp(Q, d0, gR, gQ, g1, g2);
-        dz = damp0_dz(z, i1, i2);
-        gz += dz.d1 * g1 + dz.d2 * g2;
-      } else {
-        Damp d3;
-        d3.d1 = i1;
-        d3.d2 = i2;
-        relax_vjp(Q, d3, gP, gQ, g1, g2);
-        const Damp dz = damp3_dz(i1, i2);
-        gz += dz.d1 * g1 + dz.d2 * g2;
-        double Qf[NV];
-        for (int n = 0; n < NV; n++) Qf[n] = Q[n];
-        Qf[1] += dt * force;
-        conservative_vjp(Qf, gI, gQ);
-        /* Data0 part: cons(relax(prim(U0), damp0(z0))) */
-        double U0[NV], Q0[NV], R0[NV], gR0[NV] = { 0 }, gQ0[NV] = { 0 };
-        ld(in0, pl, off, U0);
-        primitive(U0, Q0);
-        const double z0 = Q0[0] * sqrt(Q0[3]) * dtkn;
-        double j1, j2;
-        damp_inv(z0, j1, j2);
-        const Damp e0 = damp0(z0, j1, j2);
-        relax(Q0, e0, R0);
-        conservative_vjp(R0, gI, gR0);
-        double h1 = 0.0, h2 = 0.0;
-        relax_vjp(Q0, e0, gR0, gQ0, h1, h2);
-        const Damp ez = damp0_dz(z0, j1, j2);
-        z_vjp(Q0, dtkn, kn, ez.d1 * h1 + ez.d2 * h2, gQ0, gkn);
-        primitive_vjp(U0, gQ0, gU0);
-      }
-      z_vjp(Q, dtkn, kn, gz, gQ, gkn);
-      primitive_vjp(U, gQ, gU);
-    }
-    if (STAGE == 1) {
-      for (int m = 0; m < NV; m++) gin[off + m * pl] += gU[m];
-    } else {
-      for (int m = 0; m < NV; m++) {
-        gin[off + m * pl] = gU[m];
-        gin0[off + m * pl] += gU0[m];
-      }
-    }
-  }
+  if (i < B.lx * B.ly)
+    gkn = adj_cell_cell<STAGE, LIM>(in, in0, P, FxL, FxR, FyL, FyR, gout, gin, gin0, B, i % B.lx,
+                                    i / B.lx, dt, kn, force, scale);
   /* dJ/dKn: one partial sum per workgroup, fixed order */
   __shared__ double red[ADJ_THREADS];
   red[threadIdx.x] = gkn;

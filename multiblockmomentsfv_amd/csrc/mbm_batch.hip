@@ -1,0 +1,156 @@
+/*
+ * mbm_batch.hip -- many independent decks advanced together (no reference
+ * counterpart: the reference runs one deck per process).
+ *
+ * All blocks of all decks share the three layer buffers, the tile list and
+ * the ghost map; every block knows its deck. Each deck has its own Steps
+ * controller on the device (dt, rejection, t, DATA/NEW parity), so one Heun
+ * attempt of every deck is five launches (stage 1, fill, stage 2, fill,
+ * controller), captured once in a hipGraph and replayed without host round
+ * trips. Tile height is chosen by timing candidate heights on the batch.
+ */
+#include "mbm_internal.h"
+
+#include <algorithm>
+
+using namespace mbmi;
+
+extern "C" int mbm_batch_create(const mbm_block_desc *blocks, const int *nblocks_per_deck,
+                                int ndecks, const mbm_params *params, void *stream,
+                                mbm_ctx **out)
+{
+  if (!out || !blocks || !nblocks_per_deck || !params || ndecks < 1)
+    return set_err(MBM_E_ARG, "invalid argument");
+  *out = nullptr;
+  std::vector<int> first(ndecks + 1, 0);
+  for (int d = 0; d < ndecks; d++) {
+    if (nblocks_per_deck[d] < 1 || nblocks_per_deck[d] > MBM_MAX_BLOCKS)
+      return set_err(MBM_E_ARG, "deck %d: %d blocks outside [1,%d]", d, nblocks_per_deck[d], MBM_MAX_BLOCKS);
+    if (params[d].limiter != params[0].limiter || params[d].scenario != params[0].scenario)
+      return set_err(MBM_E_ARG, "deck %d: the limiter and scenario must be the same in a batch", d);
+    first[d + 1] = first[d] + nblocks_per_deck[d];
+  }
+  const int nb = first[ndecks];
+  std::vector<mbm_block_desc> glob(blocks, blocks + nb);
+  for (int d = 0; d < ndecks; d++) {
+    int rc = validate(blocks + first[d], nblocks_per_deck[d], &params[d]);
+    if (rc) return set_err(rc, "deck %d: %s", d, mbm_last_error());
+    for (int k = first[d]; k < first[d + 1]; k++)
+      for (int s = 0; s < 4; s++)
+        if (glob[k].bc[s] > 0) glob[k].bc[s] += first[d];
+  }
+  int rc;
+  mbm_ctx *c = create_ctx(glob.data(), nb, first.data(), ndecks, params, stream, &rc);
+  if (!c) return rc;
+  c->batch = true;
+  c->deck_parity.assign(ndecks, 0);
+  *out = c;
+  return MBM_OK;
+}
+
+extern "C" int mbm_batch_run(mbm_ctx *c, const double *tend, const double *cflreg, double cflmax,
+                             int maxsteps, double *t, int *nsteps, int *nrej)
+{
+  if (!c || !tend || !cflreg || !t) return set_err(MBM_E_ARG, "NULL argument");
+  if (c->deck_parity.size() != (size_t)c->ndecks) c->deck_parity.assign(c->ndecks, 0);
+  std::vector<DeckState> st(c->ndecks);
+  for (int d = 0; d < c->ndecks; d++) {
+    if (!(cflreg[d] < cflmax)) return set_err(MBM_E_ARG, "deck %d: CFL target >= CFLmax", d);
+    DeckState &S = st[d];
+    S = DeckState{};
+    S.t = t[d];
+    S.dt = 0.0001;
+    S.tend = tend[d];
+    S.cflmax = cflmax;
+    S.cflreg = cflreg[d];
+    S.kn = c->deck_par[d].kn;
+    S.force = c->deck_par[d].force;
+    S.maxsteps = maxsteps;
+    S.done = !(t[d] < tend[d]);
+    S.parity = c->deck_parity[d];
+  }
+  HIPCHK(hipMemcpyAsync(c->ddecks, st.data(), st.size() * sizeof(DeckState), hipMemcpyHostToDevice,
+                        c->stream));
+  /* roles stay fixed in a batch: each deck's parity says where its DATA is */
+  const int ndecks = c->ndecks;
+  int rc = device_steps(c, 64);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(st.data(), c->ddecks, st.size() * sizeof(DeckState), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int d = 0; d < ndecks; d++) {
+    t[d] = st[d].t;
+    if (nsteps) nsteps[d] = st[d].nstep;
+    if (nrej) nrej[d] = st[d].nrej;
+    c->deck_parity[d] = st[d].parity;
+  }
+  return MBM_OK;
+}
+
+extern "C" int mbm_batch_download(mbm_ctx *c, int deck, int block, double *host)
+{
+  if (!c || deck < 0 || deck >= c->ndecks) return set_err(MBM_E_ARG, "invalid deck");
+  const int k = c->deck_first[deck] + block;
+  if (block < 0 || k >= c->deck_first[deck + 1]) return set_err(MBM_E_ARG, "invalid block");
+  const int par = c->deck_parity.empty() ? 0 : c->deck_parity[deck];
+  const int phys = c->role[par ? MBM_L_NEW : MBM_L_DATA];
+  const BlockDev &B = c->hblk[k];
+  std::vector<double> soa((size_t)B.plane * NV);
+  HIPCHK(hipMemcpyAsync(soa.data(), c->layer[phys] + B.off, soa.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < B.ly; b++)
+    for (int a = 0; a < B.lx; a++)
+      for (int m = 0; m < NV; m++)
+        host[(size_t)NV * ((size_t)b * B.lx + a) + m] = soa[(size_t)m * B.plane + (size_t)b * B.pitch + a];
+  return MBM_OK;
+}
+
+extern "C" int mbm_batch_ndecks(const mbm_ctx *c) { return c ? c->ndecks : 0; }
+
+/* Times 'reps' stage pairs of every deck (at the decks' current dt) for
+ * each candidate tile height and keeps the fastest. The state is not
+ * changed: the stages write TMP and the NEW side only. */
+extern "C" int mbm_batch_autotune(mbm_ctx *c, const int *rows, int nrows, int reps, int *best,
+                                  double *best_ms)
+{
+  if (!c || !rows || nrows < 1 || reps < 1) return set_err(MBM_E_ARG, "invalid argument");
+  if (c->deck_parity.size() != (size_t)c->ndecks) c->deck_parity.assign(c->ndecks, 0);
+  std::vector<DeckState> st(c->ndecks);
+  for (int d = 0; d < c->ndecks; d++) {
+    DeckState &S = st[d];
+    S = DeckState{};
+    S.dt = 1e-5;
+    S.tend = 1.0;
+    S.cflmax = 1.0;
+    S.cflreg = 0.9;
+    S.kn = c->deck_par[d].kn;
+    S.force = c->deck_par[d].force;
+    S.parity = c->deck_parity[d];
+  }
+  HIPCHK(hipMemcpyAsync(c->ddecks, st.data(), st.size() * sizeof(DeckState), hipMemcpyHostToDevice,
+                        c->stream));
+  int bestr = c->tile_rows, rc;
+  double bestt = 1e300;
+  for (int i = 0; i < nrows; i++) {
+    if ((rc = mbm_set_tile_rows(c, rows[i]))) return rc;
+    if ((rc = enqueue_device_stages(c))) return rc;    /* warm-up */
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    for (int r = 0; r < reps; r++)
+      if ((rc = enqueue_device_stages(c))) return rc;
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    if (ms / reps < bestt) {
+      bestt = ms / reps;
+      bestr = rows[i];
+    }
+  }
+  if ((rc = clear_slots(c))) return rc;
+  if ((rc = mbm_set_tile_rows(c, bestr))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (best) *best = bestr;
+  if (best_ms) *best_ms = bestt;
+  return MBM_OK;
+}

@@ -100,10 +100,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int a = T.a0 + lane;
   const int ac = a < lx ? a : lx - 1;
   const long long pl = B.plane;
-  const double *__restrict__ inb = A.in + B.off + ac;
-  const double *__restrict__ in0b = (STAGE == 2) ? A.in0 + B.off + ac : nullptr;
-  double *__restrict__ outb = A.out + B.off + a;
-  const double dt = A.dt, dtkn = A.dtkn, force = A.force;
+  const double *in = A.in, *in0 = A.in0;
+  double *out = A.out;
+  double dt = A.dt, dtkn = A.dtkn, force = A.force;
+  if (A.decks) {                       /* device-controlled: per-deck dt and buffers */
+    const DeckState &S = A.decks[B.deck];
+    if (S.done) return;
+    dt = S.dt;
+    dtkn = S.dt / S.kn;
+    force = S.force;
+    const double *data = S.parity ? A.buf1 : A.buf0;
+    double *newb = const_cast<double *>(S.parity ? A.buf0 : A.buf1);
+    in = STAGE == 1 ? data : A.tmp;
+    in0 = data;
+    out = STAGE == 1 ? A.tmp : newb;
+  }
+  const double *__restrict__ inb = in + B.off + ac;
+  const double *__restrict__ in0b = (STAGE == 2) ? in0 + B.off + ac : nullptr;
+  double *__restrict__ outb = out + B.off + a;
   const double dtdx = dt / B.dx, dtdy = dt / B.dy;
 
   const bool col_ring3 = (a == 0 || a >= lx - 1);
@@ -224,15 +238,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   }
 }
 
-/* SetBoundary as a gather over the precomputed ghost map (Boundary.c:173-193) */
+/* SetBoundary as a gather over the precomputed ghost map (Boundary.c:173-193).
+ * decks != nullptr: layer chosen per deck (which 1: TMP, 2: NEW by parity),
+ * decks that are done are skipped. */
 __global__ void __launch_bounds__(256) fill_kernel(double *layer, const BlockDev *blocks,
                                                    const GhostDev *map, int n, int scenario,
-                                                   const double *tj)
+                                                   const double *tj, const DeckState *decks,
+                                                   double *buf0, double *buf1, double *tmp,
+                                                   int which)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const GhostDev g = map[i];
   const BlockDev D = blocks[g.dblk];
+  if (decks) {
+    const DeckState &S = decks[D.deck];
+    if (S.done) return;
+    layer = which == 1 ? tmp : (S.parity ? buf0 : buf1);
+  }
   double v[NV];
   if (g.sblk >= 0) {
     const BlockDev S = blocks[g.sblk];
@@ -248,6 +271,58 @@ __global__ void __launch_bounds__(256) fill_kernel(double *layer, const BlockDev
   double *dst = layer + D.off + g.dcell;
 #pragma unroll
   for (int m = 0; m < NV; m++) dst[m * D.plane] = ((g.sign >> m) & 1) ? -v[m] : v[m];
+}
+
+/* Steps controller of every deck (Main.c:202-216) after one TimeUpdate:
+ * combined CFL (Main.c:395, :268-282), accept/reject, dt update, clamp to
+ * tend, parity flip instead of the New->Data copy; clears the slots. */
+__global__ void control_kernel(DeckState *decks, unsigned long long *slots,
+                               const BlockDev *blocks, const int *first, int ndecks, int nb,
+                               RecordDev *rec)
+{
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndecks) return;
+  DeckState S = decks[d];
+  if (S.done) return;
+  double c1 = 0.0, c2 = 0.0;
+  for (int st = 0; st < 2; st++)
+    for (int k = first[d]; k < first[d + 1]; k++) {
+      unsigned long long *s = slots + (size_t)3 * (st * nb + k);
+      const double cx = __longlong_as_double((long long)s[0]);
+      const double cy = __longlong_as_double((long long)s[1]);
+      const double cp = __longlong_as_double((long long)s[2]);
+      const double v = S.dt * cx / blocks[k].dx + S.dt * cy / blocks[k].dy + cp;
+      if (st == 0) { if (v > c1) c1 = v; }
+      else { if (v > c2) c2 = v; }
+      s[0] = 0ull; s[1] = 0ull; s[2] = 0ull;
+    }
+  const double m2 = c2 > 0.0 ? c2 : 0.0;
+  const double cfl = c1 > m2 ? c1 : m2;
+  const double used = S.dt;
+  const int acc = !(cfl > S.cflmax);
+  if (!acc) {
+    S.dt = S.dt * S.cflreg / cfl;
+    S.nrej++;
+  } else {
+    S.t += S.dt;
+    S.dt = S.dt * S.cflreg / cfl;
+    if (S.t + S.dt > S.tend) S.dt = S.tend - S.t;
+    S.nstep++;
+    S.parity ^= 1;
+    if (!(S.t < S.tend) || (S.maxsteps > 0 && S.nstep >= S.maxsteps)) S.done = 1;
+  }
+  S.cfl = cfl;
+  if (rec && d == 0 && S.nrec < S.reccap) {
+    RecordDev r;
+    r.cfl = cfl;
+    r.dt = used;
+    r.t = S.t;
+    r.accepted = acc;
+    r.step = S.nstep + (acc ? 0 : 1);
+    rec[S.nrec] = r;
+  }
+  S.nrec++;
+  decks[d] = S;
 }
 
 /* Combined CFL (Main.c:395 per block, Main.c:268-282 over blocks and stages),
@@ -543,7 +618,8 @@ int fill_buffer(mbm_ctx *c, double *layer)
   if (c->nghost == 0) return MBM_OK;
   const int nblk = (c->nghost + 255) / 256;
   hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, layer, c->dblk, c->dghost,
-                     c->nghost, c->par.scenario, c->dtj);
+                     c->nghost, c->par.scenario, c->dtj, (const DeckState *)nullptr,
+                     (double *)nullptr, (double *)nullptr, (double *)nullptr, 0);
   HIPCHK(hipGetLastError());
   return MBM_OK;
 }
@@ -576,6 +652,209 @@ int upload_params(mbm_ctx *c)
 
 } // namespace mbmi
 
+namespace mbmi {
+
+/* Device state of one or more decks whose blocks are listed back to back;
+ * deck d owns blocks [deck_first[d], deck_first[d+1]) and its sync codes are
+ * already global (1-based over all blocks). */
+mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_first, int ndecks,
+                    const mbm_params *deck_par, void *stream, int *rcp)
+{
+  int &rc = *rcp;
+  rc = MBM_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    rc = set_err(MBM_E_NODEV, "no HIP device available");
+    return nullptr;
+  }
+  mbm_ctx *c = new mbm_ctx();
+  auto fail = [&](int code) { rc = code; mbm_destroy(c); return (mbm_ctx *)nullptr; };
+  auto hipok = [&](hipError_t e, const char *what) {
+    if (e != hipSuccess) rc = set_err(MBM_E_HIP, "%s failed: %s", what, hipGetErrorString(e));
+    return e == hipSuccess;
+  };
+  c->nb = nblocks;
+  c->desc.assign(blocks, blocks + nblocks);
+  c->par = deck_par[0];
+  c->tile_rows = 32;
+  c->ndecks = ndecks;
+  c->deck_first.assign(deck_first, deck_first + ndecks + 1);
+  c->deck_par.assign(deck_par, deck_par + ndecks);
+  if (!hipok(hipGetDevice(&c->device), "hipGetDevice")) return fail(rc);
+  size_t off = 0;
+  for (int d = 0; d < ndecks; d++)
+    for (int k = deck_first[d]; k < deck_first[d + 1]; k++) {
+      BlockDev b{};
+      b.lx = blocks[k].nx + 2 * MBC;
+      b.ly = blocks[k].ny + 2 * MBC;
+      b.pitch = (b.lx + 7) & ~7;
+      b.plane = (long long)b.ly * b.pitch;
+      b.plane = (b.plane + 31) & ~31ll;
+      b.off = (long long)off;
+      b.deck = d;
+      geometry(blocks[k], b.bx, b.by, b.dx, b.dy);
+      off += (size_t)b.plane * NV;
+      c->hblk.push_back(b);
+    }
+  c->layer_elems = off;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+  } else {
+    if (!hipok(hipStreamCreate(&c->stream), "hipStreamCreate")) return fail(rc);
+    c->own_stream = true;
+  }
+  for (int l = 0; l < 3; l++) {
+    if (hipMalloc(&c->layer[l], off * sizeof(double)) != hipSuccess) {
+      c->layer[l] = nullptr;
+      return fail(set_err(MBM_E_NOMEM, "device allocation of %zu bytes failed", off * sizeof(double)));
+    }
+    if (!hipok(hipMemsetAsync(c->layer[l], 0, off * sizeof(double), c->stream), "hipMemsetAsync"))
+      return fail(rc);
+    c->role[l] = l;
+  }
+  std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
+  c->nghost = (int)gm.size();
+  const bool ok =
+      hipok(hipMalloc(&c->dblk, nblocks * sizeof(BlockDev)), "hipMalloc") &&
+      hipok(hipMemcpy(c->dblk, c->hblk.data(), nblocks * sizeof(BlockDev), hipMemcpyHostToDevice), "hipMemcpy") &&
+      hipok(hipMalloc(&c->dghost, std::max<size_t>(1, gm.size()) * sizeof(GhostDev)), "hipMalloc") &&
+      (gm.empty() || hipok(hipMemcpy(c->dghost, gm.data(), gm.size() * sizeof(GhostDev), hipMemcpyHostToDevice), "hipMemcpy")) &&
+      hipok(hipMalloc(&c->dslots, (size_t)6 * nblocks * sizeof(unsigned long long)), "hipMalloc") &&
+      hipok(hipMemsetAsync(c->dslots, 0, (size_t)6 * nblocks * sizeof(unsigned long long), c->stream), "hipMemsetAsync") &&
+      hipok(hipMalloc(&c->dcfl, sizeof(double)), "hipMalloc") &&
+      hipok(hipMalloc(&c->dtj, 5 * sizeof(double)), "hipMalloc") &&
+      hipok(hipHostMalloc(&c->hcfl, sizeof(double)), "hipHostMalloc") &&
+      hipok(hipMalloc(&c->ddeck_first, (ndecks + 1) * sizeof(int)), "hipMalloc") &&
+      hipok(hipMemcpy(c->ddeck_first, deck_first, (ndecks + 1) * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy") &&
+      hipok(hipMalloc(&c->ddecks, ndecks * sizeof(DeckState)), "hipMalloc");
+  if (!ok) return fail(rc);
+  for (int i = 0; i < 4; i++)
+    if (!hipok(hipEventCreate(&c->ev[i]), "hipEventCreate")) return fail(rc);
+  if ((rc = upload_params(c)) || (rc = upload_tiles(c))) return fail(rc);
+  if (!hipok(hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return fail(rc);
+  return c;
+}
+
+void release_graphs(mbm_ctx *c)
+{
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      if (c->graph_ok[i][j]) {
+        (void)hipGraphExecDestroy(c->graph[i][j]);
+        c->graph_ok[i][j] = false;
+      }
+}
+
+namespace {
+
+/* one device-controlled attempt of every deck: 2 stages, 2 fills, controller */
+int enqueue_device_step(mbm_ctx *c, double *buf0, double *buf1, double *tmp)
+{
+  StageArgs A{};
+  A.blocks = c->dblk;
+  A.tiles = c->dtiles;
+  A.ntiles = c->ntiles;
+  A.decks = c->ddecks;
+  A.buf0 = buf0;
+  A.buf1 = buf1;
+  A.tmp = tmp;
+  A.slots = c->dslots;
+  int rc;
+  if ((rc = launch_stage<1>(c, A))) return rc;
+  const int nblk = (c->nghost + 255) / 256;
+  if (c->nghost)
+    hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, (double *)nullptr, c->dblk,
+                       c->dghost, c->nghost, c->par.scenario, c->dtj, (const DeckState *)c->ddecks,
+                       buf0, buf1, tmp, 1);
+  A.slots = c->dslots + 3 * c->nb;
+  if ((rc = launch_stage<2>(c, A))) return rc;
+  if (c->nghost)
+    hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, (double *)nullptr, c->dblk,
+                       c->dghost, c->nghost, c->par.scenario, c->dtj, (const DeckState *)c->ddecks,
+                       buf0, buf1, tmp, 2);
+  hipLaunchKernelGGL(control_kernel, dim3((c->ndecks + 63) / 64), dim3(64), 0, c->stream, c->ddecks,
+                     c->dslots, c->dblk, c->ddeck_first, c->ndecks, c->nb, c->drec);
+  HIPCHK(hipGetLastError());
+  return MBM_OK;
+}
+
+} // namespace
+
+/* stages and fills of every deck at its current dt, without the controller
+ * (writes TMP and NEW only; used to time tile shapes) */
+int enqueue_device_stages(mbm_ctx *c)
+{
+  StageArgs A{};
+  A.blocks = c->dblk;
+  A.tiles = c->dtiles;
+  A.ntiles = c->ntiles;
+  A.decks = c->ddecks;
+  A.buf0 = c->layer[c->role[MBM_L_DATA]];
+  A.buf1 = c->layer[c->role[MBM_L_NEW]];
+  A.tmp = c->layer[c->role[MBM_L_TMP]];
+  A.slots = c->dslots;
+  int rc;
+  if ((rc = launch_stage<1>(c, A))) return rc;
+  A.slots = c->dslots + 3 * c->nb;
+  return launch_stage<2>(c, A);
+}
+
+int clear_slots(mbm_ctx *c)
+{
+  const int n = 6 * c->nb;
+  hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->dslots, n);
+  HIPCHK(hipGetLastError());
+  return MBM_OK;
+}
+
+/* Runs the controllers of all decks on the device until every deck is done:
+ * the step (2 stages, 2 fills, controller) is captured once in a hipGraph
+ * and replayed in chunks; the host reads the deck states between chunks.
+ * The current DATA/NEW roles are buf0/buf1 at parity 0; on return the roles
+ * of the physical buffers are updated from deck 0's parity (a plain context
+ * has one deck; batches keep per-deck parity in ddecks). */
+int device_steps(mbm_ctx *c, int maxchunk)
+{
+  const int iD = c->role[MBM_L_DATA], iN = c->role[MBM_L_NEW], iT = c->role[MBM_L_TMP];
+  double *buf0 = c->layer[iD], *buf1 = c->layer[iN], *tmp = c->layer[iT];
+  if (!c->graph_ok[iD][iN]) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_device_step(c, buf0, buf1, tmp);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_err(MBM_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph[iD][iN], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return set_err(MBM_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    c->graph_ok[iD][iN] = true;
+  }
+  std::vector<DeckState> st(c->ndecks);
+  HIPCHK(hipMemcpyAsync(st.data(), c->ddecks, c->ndecks * sizeof(DeckState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (;;) {
+    int remaining = 0;
+    for (const DeckState &S : st)
+      if (!S.done) {
+        /* attempts left: bounded by maxsteps, estimated from (tend-t)/dt */
+        double est = S.dt > 0 ? (S.tend - S.t) / S.dt : 1.0;
+        int r = est > 1e6 ? 1000000 : (int)est + 1;
+        if (S.maxsteps > 0) r = std::min(r, S.maxsteps - S.nstep);
+        remaining = std::max(remaining, std::max(r, 1));
+      }
+    if (remaining == 0) break;
+    const int chunk = std::min(remaining, maxchunk);
+    for (int i = 0; i < chunk; i++) HIPCHK(hipGraphLaunch(c->graph[iD][iN], c->stream));
+    HIPCHK(hipMemcpyAsync(st.data(), c->ddecks, c->ndecks * sizeof(DeckState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (!c->batch && st[0].parity) std::swap(c->role[MBM_L_DATA], c->role[MBM_L_NEW]);
+  return MBM_OK;
+}
+
+} // namespace mbmi
+
 extern "C" int mbm_create(const mbm_block_desc *blocks, int nblocks, const mbm_params *p,
                           void *stream, mbm_ctx **out)
 {
@@ -583,60 +862,9 @@ extern "C" int mbm_create(const mbm_block_desc *blocks, int nblocks, const mbm_p
   *out = nullptr;
   int rc = validate(blocks, nblocks, p);
   if (rc) return rc;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-    return set_err(MBM_E_NODEV, "no HIP device available");
-  mbm_ctx *c = new mbm_ctx();
-  c->nb = nblocks;
-  c->desc.assign(blocks, blocks + nblocks);
-  c->par = *p;
-  c->tile_rows = 32;
-  HIPCHK(hipGetDevice(&c->device));
-  size_t off = 0;
-  for (int k = 0; k < nblocks; k++) {
-    BlockDev b{};
-    b.lx = blocks[k].nx + 2 * MBC;
-    b.ly = blocks[k].ny + 2 * MBC;
-    b.pitch = (b.lx + 7) & ~7;
-    b.plane = (long long)b.ly * b.pitch;
-    b.plane = (b.plane + 31) & ~31ll;
-    b.off = (long long)off;
-    geometry(blocks[k], b.bx, b.by, b.dx, b.dy);
-    off += (size_t)b.plane * NV;
-    c->hblk.push_back(b);
-  }
-  c->layer_elems = off;
-  if (stream) {
-    c->stream = (hipStream_t)stream;
-    c->own_stream = false;
-  } else {
-    HIPCHK(hipStreamCreate(&c->stream));
-    c->own_stream = true;
-  }
-  for (int l = 0; l < 3; l++) {
-    if (hipMalloc(&c->layer[l], off * sizeof(double)) != hipSuccess) {
-      mbm_destroy(c);
-      return set_err(MBM_E_NOMEM, "device allocation of %zu bytes failed", off * sizeof(double));
-    }
-    HIPCHK(hipMemsetAsync(c->layer[l], 0, off * sizeof(double), c->stream));
-    c->role[l] = l;
-  }
-  HIPCHK(hipMalloc(&c->dblk, nblocks * sizeof(BlockDev)));
-  HIPCHK(hipMemcpy(c->dblk, c->hblk.data(), nblocks * sizeof(BlockDev), hipMemcpyHostToDevice));
-  std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
-  c->nghost = (int)gm.size();
-  HIPCHK(hipMalloc(&c->dghost, std::max<size_t>(1, gm.size()) * sizeof(GhostDev)));
-  HIPCHK(hipMemcpy(c->dghost, gm.data(), gm.size() * sizeof(GhostDev), hipMemcpyHostToDevice));
-  HIPCHK(hipMalloc(&c->dslots, (size_t)6 * nblocks * sizeof(unsigned long long)));
-  HIPCHK(hipMemsetAsync(c->dslots, 0, (size_t)6 * nblocks * sizeof(unsigned long long), c->stream));
-  HIPCHK(hipMalloc(&c->dcfl, sizeof(double)));
-  HIPCHK(hipMalloc(&c->dtj, 5 * sizeof(double)));
-  HIPCHK(hipHostMalloc(&c->hcfl, sizeof(double)));
-  for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&c->ev[i]));
-  rc = upload_params(c);
-  if (!rc) rc = upload_tiles(c);
-  if (rc) { mbm_destroy(c); return rc; }
-  HIPCHK(hipStreamSynchronize(c->stream));
+  const int first[2] = { 0, nblocks };
+  mbm_ctx *c = create_ctx(blocks, nblocks, first, 1, p, stream, &rc);
+  if (!c) return rc;
   *out = c;
   return MBM_OK;
 }
@@ -645,6 +873,8 @@ extern "C" int mbm_destroy(mbm_ctx *c)
 {
   if (!c) return MBM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->adj) mbm_adj_release(c);
+  release_graphs(c);
   for (int l = 0; l < 3; l++) if (c->layer[l]) (void)hipFree(c->layer[l]);
   if (c->dblk) (void)hipFree(c->dblk);
   if (c->dtiles) (void)hipFree(c->dtiles);
@@ -653,6 +883,9 @@ extern "C" int mbm_destroy(mbm_ctx *c)
   if (c->dcfl) (void)hipFree(c->dcfl);
   if (c->dtj) (void)hipFree(c->dtj);
   if (c->hcfl) (void)hipHostFree(c->hcfl);
+  if (c->ddeck_first) (void)hipFree(c->ddeck_first);
+  if (c->ddecks) (void)hipFree(c->ddecks);
+  if (c->drec) (void)hipFree(c->drec);
   for (int i = 0; i < 4; i++) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -663,6 +896,7 @@ extern "C" int mbm_set_stream(mbm_ctx *c, void *stream)
 {
   if (!c) return set_err(MBM_E_ARG, "NULL context");
   HIPCHK(hipStreamSynchronize(c->stream));
+  release_graphs(c);
   if (c->own_stream) HIPCHK(hipStreamDestroy(c->stream));
   if (stream) { c->stream = (hipStream_t)stream; c->own_stream = false; }
   else { HIPCHK(hipStreamCreate(&c->stream)); c->own_stream = true; }
@@ -674,7 +908,9 @@ extern "C" int mbm_set_params(mbm_ctx *c, const mbm_params *p)
   if (!c) return set_err(MBM_E_ARG, "NULL context");
   int rc = validate(c->desc.data(), c->nb, p);
   if (rc) return rc;
+  if (p->limiter != c->par.limiter) release_graphs(c);
   c->par = *p;
+  for (auto &dp : c->deck_par) dp = *p;
   return upload_params(c);
 }
 
@@ -684,6 +920,7 @@ extern "C" int mbm_set_tile_rows(mbm_ctx *c, int rows)
   if (rows < 1 || rows > 4096) return set_err(MBM_E_ARG, "tile rows %d outside [1,4096]", rows);
   c->tile_rows = rows;
   HIPCHK(hipStreamSynchronize(c->stream));
+  release_graphs(c);
   return upload_tiles(c);
 }
 
@@ -691,6 +928,7 @@ extern "C" int mbm_init(mbm_ctx *c)
 {
   if (!c) return set_err(MBM_E_ARG, "NULL context");
   c->role[0] = 0; c->role[1] = 1; c->role[2] = 2;
+  std::fill(c->deck_parity.begin(), c->deck_parity.end(), 0);
   for (int l = 0; l < 3; l++)
     HIPCHK(hipMemsetAsync(c->layer[l], 0, c->layer_elems * sizeof(double), c->stream));
   for (int k = 0; k < c->nb; k++) {
@@ -814,8 +1052,12 @@ int read_cfl(mbm_ctx *c, double *cfl)
     float ms1 = 0.f, ms2 = 0.f;
     HIPCHK(hipEventElapsedTime(&ms1, c->ev[0], c->ev[1]));
     HIPCHK(hipEventElapsedTime(&ms2, c->ev[2], c->ev[3]));
+    float msf = 0.f;
+    HIPCHK(hipEventElapsedTime(&msf, c->ev[1], c->ev[2]));
     c->stage_ms += ms1 + ms2;
     c->nstage += 2;
+    c->fill_ms += msf;            /* the TMP fill between the two stages */
+    c->nfill += 1;
   }
   *cfl = *c->hcfl;
   return MBM_OK;
@@ -844,38 +1086,51 @@ extern "C" int mbm_steps(mbm_ctx *c, double *t, double tend, double cflmax, doub
                          int *nsteps, int *nrej)
 {
   if (!c || !t) return set_err(MBM_E_ARG, "NULL argument");
-  int nstep = 0, rej = 0, nr = 0, rc;
-  double cfl, dt = (dt_io && *dt_io > 0.0) ? *dt_io : 0.0001;
-  while (*t < tend) {
-    if (maxsteps > 0 && nstep >= maxsteps) break;
-    nstep++;
-    do {
-      const double dt_used = dt;
-      if ((rc = mbm_time_update(c, dt, *t, &cfl))) return rc;
-      const int acc = !(cfl > cflmax);
-      if (!acc) {
-        dt = dt * cflreg / cfl;
-        rej++;
-      } else {
-        *t += dt;
-        dt = dt * cflreg / cfl;
-        if (*t + dt > tend) dt = tend - (*t);
-      }
-      if (rec && nr < reccap) {
-        rec[nr].cfl = cfl;
-        rec[nr].dt = dt_used;
-        rec[nr].t = *t;
-        rec[nr].accepted = acc;
-        rec[nr].step = nstep;
-      }
-      nr++;
-    } while (cfl > cflmax);
-    mbm_commit(c);
+  if (c->batch) return set_err(MBM_E_STATE, "mbm_steps needs a plain (single-deck) context");
+  if (!(cflreg < cflmax)) return set_err(MBM_E_ARG, "CFL target %g >= CFLmax %g never accepts", cflreg, cflmax);
+  if (rec && reccap > c->reccap) {
+    if (c->drec) HIPCHK(hipFree(c->drec));
+    c->drec = nullptr;
+    HIPCHK(hipMalloc(&c->drec, (size_t)reccap * sizeof(RecordDev)));
+    c->reccap = reccap;
+    release_graphs(c);   /* the controller's record pointer is baked into the graph */
   }
-  if (nrec) *nrec = nr;
-  if (nsteps) *nsteps = nstep;
-  if (nrej) *nrej = rej;
-  if (dt_io) *dt_io = dt;
+  if (!c->drec) {
+    HIPCHK(hipMalloc(&c->drec, sizeof(RecordDev)));
+    c->reccap = 1;
+  }
+  DeckState S{};
+  S.t = *t;
+  S.dt = (dt_io && *dt_io > 0.0) ? *dt_io : 0.0001;
+  S.tend = tend;
+  S.cflmax = cflmax;
+  S.cflreg = cflreg;
+  S.kn = c->par.kn;
+  S.force = c->par.force;
+  S.maxsteps = maxsteps;
+  S.done = !(*t < tend);
+  S.reccap = rec ? reccap : 0;
+  HIPCHK(hipMemcpyAsync(c->ddecks, &S, sizeof S, hipMemcpyHostToDevice, c->stream));
+  int rc = device_steps(c, 64);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(&S, c->ddecks, sizeof S, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (rec && S.nrec > 0) {
+    std::vector<RecordDev> r(std::min(S.nrec, reccap));
+    HIPCHK(hipMemcpy(r.data(), c->drec, r.size() * sizeof(RecordDev), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < r.size(); i++) {
+      rec[i].cfl = r[i].cfl;
+      rec[i].dt = r[i].dt;
+      rec[i].t = r[i].t;
+      rec[i].accepted = r[i].accepted;
+      rec[i].step = r[i].step;
+    }
+  }
+  *t = S.t;
+  if (nrec) *nrec = S.nrec;
+  if (nsteps) *nsteps = S.nstep;
+  if (nrej) *nrej = S.nrej;
+  if (dt_io) *dt_io = S.dt;
   return MBM_OK;
 }
 

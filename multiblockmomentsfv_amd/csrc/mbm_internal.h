@@ -35,8 +35,23 @@ constexpr int WAVES_PER_WG = 4;
 struct BlockDev {
   long long off;      /* element offset of block in a layer buffer */
   long long plane;    /* elements between components */
-  int lx, ly, pitch, pad;
+  int lx, ly, pitch, deck;   /* deck: index of the deck the block belongs to */
   double dx, dy, bx, by;
+};
+
+/* Device-resident state of the Steps controller of one deck (Main.c:189-244). */
+struct DeckState {
+  double t, dt, tend, cflmax, cflreg, kn, force;
+  double cfl;         /* combined CFL of the last attempt */
+  int nstep, nrej, maxsteps, done;
+  int parity;         /* 0: DATA in buf0, NEW in buf1; 1: swapped */
+  int nrec, reccap, pad;
+};
+
+/* one attempt of the controller (mirrors mbm_step_record) */
+struct RecordDev {
+  double cfl, dt, t;
+  int accepted, step;
 };
 
 struct TileDev {
@@ -64,6 +79,11 @@ struct StageArgs {
   unsigned long long *slots;   /* per block: max x-face speed, y-face speed, prod CFL */
   int ntiles;
   double dt, dtkn, force;
+  /* device-controlled mode (decks != nullptr): dt, Kn and F come from the
+   * block's deck state, DATA/NEW from its parity; in/in0/out are unused. */
+  const DeckState *decks;
+  const double *buf0, *buf1;
+  double *tmp;
 };
 
 
@@ -72,6 +92,8 @@ struct StageArgs {
 using mbmi::BlockDev;
 using mbmi::TileDev;
 using mbmi::GhostDev;
+using mbmi::DeckState;
+using mbmi::RecordDev;
 
 struct mbm_ctx {
   int nb;
@@ -99,6 +121,19 @@ struct mbm_ctx {
   int nstage, nfill;
   int device;
   mbm_adj_state *adj;         /* adjoint buffers, created on demand */
+  /* decks (one for a plain context): blocks [first[d], first[d+1]) */
+  int ndecks;
+  std::vector<int> deck_first;
+  std::vector<mbm_params> deck_par;
+  int *ddeck_first;
+  DeckState *ddecks;
+  RecordDev *drec;
+  int reccap;
+  bool batch;                     /* created by mbm_batch_create: roles never rotate */
+  std::vector<int> deck_parity;   /* batches: DATA of deck d is buffer role[DATA] or role[NEW] */
+  /* captured Heun step + controller, per (DATA, NEW) buffer assignment */
+  hipGraphExec_t graph[3][3];
+  bool graph_ok[3][3];
 };
 
 namespace mbmi {
@@ -108,6 +143,14 @@ int do_fill(mbm_ctx *c, int layer);
 int fill_buffer(mbm_ctx *c, double *layer);
 int enqueue_stage(mbm_ctx *c, int stage, const double *in, const double *in0, double *out,
                   double dt, unsigned long long *slots);
+mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_first, int ndecks,
+                    const mbm_params *deck_par, void *stream, int *rc);
+int device_steps(mbm_ctx *c, int maxchunk);
+int upload_tiles(mbm_ctx *c);
+int validate(const mbm_block_desc *blocks, int nb, const mbm_params *p);
+int enqueue_device_stages(mbm_ctx *c);
+int clear_slots(mbm_ctx *c);
+void release_graphs(mbm_ctx *c);
 } // namespace mbmi
 
 #endif

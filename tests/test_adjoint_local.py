@@ -170,3 +170,80 @@ def test_relax_and_damping_vjp(pc):
             pc.pc_damp(flag, z - h, p(b))
             pc.pc_damp_dz(flag, z, p(dz))
             assert np.allclose((a - b) / (2 * h), dz, rtol=1e-6, atol=1e-9), (flag, z)
+
+
+def stage_fixture(seed, nx=7, ny=6):
+    """One block with a generic smooth state (every primitive varies)."""
+    import oracle
+    rng = np.random.default_rng(seed)
+    blocks = [dict(nx=nx, ny=ny, xmin=-0.5, xmax=0.6, ymin=-0.4, ymax=0.5, bc=[0, 0, 0, 0])]
+    lx, ly = nx + 6, ny + 6
+    a, b = np.meshgrid(np.arange(lx), np.arange(ly))
+    def field(s):
+        return s * np.sin(rng.uniform(0.3, 0.9) * a + rng.uniform(0.3, 0.9) * b + rng.uniform(0, 6))
+    th = 1.0 + field(0.3)
+    P = [1.0 + field(0.2), field(0.3), field(0.3), th, th + field(0.1), field(0.1), th + field(0.1),
+         field(0.05), field(0.05)]
+    U = np.empty((ly, lx, 9))
+    Pc = np.stack(P, axis=-1)
+    for j in range(ly):
+        for i in range(lx):
+            oracle.lib().orc_conservative(Pc[j, i].ctypes.data, U[j, i].ctypes.data)
+    return blocks, U
+
+
+@pytest.mark.parametrize("stage", [1, 2])
+@pytest.mark.parametrize("lim", [0, 1, 2, 3, 4])
+def test_stage_vjp_against_oracle_update(pc, stage, lim):
+    """S^T of one stage (the GPU kernels' per-cell code, run on the CPU)
+    against central differences of the oracle's Update (Main.c:297-396)."""
+    import oracle
+    pc.pc_stage_vjp.restype = ctypes.c_double
+    pc.pc_stage_vjp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double, ctypes.c_double] + [ctypes.c_void_p] * 5 + \
+                               [ctypes.c_double] * 3
+    blocks, U = stage_fixture(stage * 10 + lim)
+    _, U0 = stage_fixture(99)
+    nx, ny = blocks[0]["nx"], blocks[0]["ny"]
+    kn, dt, force = 0.02, 1e-3, 0.3
+    orc = oracle.Oracle(blocks, kn=kn, force=force, limiter=lim)
+    bx, by, dx, dy = (blocks[0]["xmax"] - blocks[0]["xmin"]) / nx, 0, 0, 0
+    dx = (blocks[0]["xmax"] - blocks[0]["xmin"]) / nx
+    dy = (blocks[0]["ymax"] - blocks[0]["ymin"]) / ny
+    rng = np.random.default_rng(7)
+    g = np.zeros_like(U)
+    g[3:-3, 3:-3] = rng.normal(size=(ny, nx, 9))
+
+    def fwd(Uin, Uin0):
+        if stage == 1:
+            orc.set([Uin], layer=0)
+            oracle.lib().orc_update_block(orc.h, 0, dt, 0.0, 0)
+            return orc.get(1)[0]
+        orc.set([Uin], layer=1)
+        orc.set([Uin0], layer=0)
+        oracle.lib().orc_update_block(orc.h, 0, dt, 0.0, 1)
+        return orc.get(2)[0]
+
+    soa = lambda A: np.ascontiguousarray(A.transpose(2, 0, 1))
+    gin = np.zeros((9, ny + 6, nx + 6))
+    gin0 = np.zeros((9, ny + 6, nx + 6))
+    gkn = pc.pc_stage_vjp(stage, lim, nx, ny, dx, dy, p(soa(U)), p(soa(U0)), p(soa(g)), p(gin), p(gin0),
+                          dt, kn, force)
+    gin = gin.transpose(1, 2, 0)
+    gin0 = gin0.transpose(1, 2, 0)
+    for trial in range(3):
+        v = rng.normal(size=U.shape) * (np.abs(U) + 1e-2)
+        h = 1e-6
+        fd = (np.sum(g * fwd(U + h * v, U0)) - np.sum(g * fwd(U - h * v, U0))) / (2 * h)
+        assert abs(np.sum(gin * v) - fd) <= 1e-6 * abs(fd) + 1e-9, ("in", trial, np.sum(gin * v), fd)
+        if stage == 2:
+            fd0 = (np.sum(g * fwd(U, U0 + h * v)) - np.sum(g * fwd(U, U0 - h * v))) / (2 * h)
+            assert abs(np.sum(gin0 * v) - fd0) <= 1e-6 * abs(fd0) + 1e-9, ("in0", trial, np.sum(gin0 * v), fd0)
+    orc2 = oracle.Oracle(blocks, kn=kn * (1 + 1e-5), force=force, limiter=lim)
+    orc3 = oracle.Oracle(blocks, kn=kn * (1 - 1e-5), force=force, limiter=lim)
+    vals = []
+    for o in (orc2, orc3):
+        orc = o
+        vals.append(np.sum(g * fwd(U, U0)))
+    fdk = (vals[0] - vals[1]) / (2e-5 * kn)
+    assert abs(gkn - fdk) <= 1e-6 * abs(fdk) + 1e-9, (gkn, fdk)

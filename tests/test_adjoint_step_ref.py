@@ -1,0 +1,87 @@
+"""The CPU reference of the adjoint of one Heun step (tests/adjoint_ref.py:
+the engine's per-cell adjoint code compiled for the host, composed with the
+transpose of the ghost fill) against central differences of the oracle's
+TimeUpdate (Main.c:253-284) with SetBoundary (Boundary.c:173-193) in front.
+
+Limiters 'none' and 'average' are smooth, so finite differences measure the
+derivative; the kinked limiters are covered per stage in
+test_adjoint_local.py and through the GPU-vs-reference comparison.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import deck_path
+
+import oracle
+
+LIBP = os.path.join(os.path.dirname(os.path.abspath(__file__)), "native", "libphyscheck.so")
+pytestmark = pytest.mark.skipif(not os.path.exists(LIBP), reason="tests/native/libphyscheck.so not built")
+
+LIMS = {"none": 0, "average": 1}
+
+
+def generic_blocks(deck, seed):
+    """Smooth state in every primitive on all cells of every block."""
+    from multiblockmomentsfv_amd.adjoint import smooth_fields
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, b in enumerate(deck.blocks):
+        bx, by, dx, dy = deck.geom(k)
+        X, Y = np.meshgrid(bx + dx * np.arange(b["nx"] + 6), by + dy * np.arange(b["ny"] + 6))
+        dr, v1, v2, d11, d22, t12, q1, q2 = smooth_fields(X, Y, rng, 1.0)
+        th = 1.0 + 0.5 * (1 - np.tanh((np.hypot(X, Y) - 0.4) / 0.1))
+        P = np.stack([1 + dr, v1, v2, th, th + d11, t12, th + d22, q1, q2], -1)
+        U = np.empty_like(P)
+        for j in range(P.shape[0]):
+            for i in range(P.shape[1]):
+                oracle.lib().orc_conservative(P[j, i].ctypes.data, U[j, i].ctypes.data)
+        out.append(U)
+    return out, rng
+
+
+def interior(a):
+    b = np.zeros_like(a)
+    b[3:-3, 3:-3] = a[3:-3, 3:-3]
+    return b
+
+
+@pytest.mark.parametrize("name", ["box_small", "lshape_small", "periodic", "mixed_bc"])
+@pytest.mark.parametrize("lim", ["none", "average"])
+def test_step_adjoint_reference_vs_oracle_fd(mbm, name, lim):
+    import adjoint_ref as ar
+    deck = mbm.read_deck(deck_path(name))
+    fmap = ar.fill_transpose_map(deck.blocks)
+    geoms = [deck.geom(k) for k in range(deck.nblocks)]
+    kn, dt = 0.01, 1e-3
+    orc = oracle.Oracle(deck.blocks, kn=kn, limiter=lim, scenario="tempjump")
+    st, rng = generic_blocks(deck, seed=len(name))
+    orc.set(st, 0)
+    orc.set_boundary(0, 0.0)
+    D = orc.get(0)
+    w = [interior(rng.normal(size=a.shape)) for a in D]
+
+    def J(s, o=orc):
+        o.set(s, 0)
+        o.set_boundary(0, 0.0)
+        o.time_update(dt, 0.0)
+        return sum(np.sum(a * b) for a, b in zip(w, o.get(2)))
+
+    gD, gk = ar.step_adjoint(orc, deck.blocks, geoms, LIMS[lim], D, w, dt, kn, 0.0, fmap)
+    gD = ar.fill_T(gD, fmap)
+    for a in gD:      # dJ/d(ghosts) is folded into the interior
+        assert np.all(a[:3] == 0) and np.all(a[-3:] == 0)
+        assert np.all(a[:, :3] == 0) and np.all(a[:, -3:] == 0)
+    for trial in range(2):
+        v = [interior(rng.normal(size=a.shape) * (np.abs(a) + 1e-2)) for a in D]
+        h = 1e-6
+        fd = (J([s + h * d for s, d in zip(D, v)]) - J([s - h * d for s, d in zip(D, v)])) / (2 * h)
+        ad = sum(np.sum(a * b) for a, b in zip(gD, v))
+        assert abs(ad - fd) <= 1e-7 * abs(fd), (trial, ad, fd)
+    vals = []
+    for f in (1 + 1e-5, 1 - 1e-5):
+        o = oracle.Oracle(deck.blocks, kn=kn * f, limiter=lim, scenario="tempjump")
+        vals.append(J(D, o))
+    fdk = (vals[0] - vals[1]) / (2e-5 * kn)
+    assert abs(gk - fdk) <= 1e-6 * abs(fdk), (gk, fdk)

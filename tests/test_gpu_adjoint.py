@@ -66,7 +66,78 @@ def run(eng, U, dts, kn=None, stride=None):
     return eng.layer_get(0)
 
 
+def gpu_to_blocks(eng, U):
+    out = []
+    for (off, plane, pitch), b in zip(eng.layer_index(), eng.blocks):
+        ly, lx = b["ny"] + 6, b["nx"] + 6
+        out.append(np.ascontiguousarray(np.stack(
+            [U[off + c * plane: off + c * plane + ly * pitch].reshape(ly, pitch)[:, :lx]
+             for c in range(9)], axis=-1)))
+    return out
+
+
+@pytest.mark.parametrize("name", ["box_small", "lshape_small", "periodic", "mixed_bc"])
 @pytest.mark.parametrize("limiter", ["harmonic", "minmod", "superbee", "average", "none"])
+def test_one_step_adjoint_matches_cpu_reference(mbm, name, limiter):
+    """GPU adjoint of one step == the CPU evaluation of the same adjoint code
+    (tests/adjoint_ref.py, itself checked against central differences of the
+    oracle), cell by cell."""
+    import adjoint_ref as ar
+    deck = mbm.read_deck(deck_path(name))
+    lim = mbm.LIMITERS[limiter]
+    eng = mbm.Engine(deck.blocks, kn=0.01, limiter=limiter, scenario="tempjump")
+    eng.init()
+    U0 = generic_state(mbm, eng, deck)
+    mask = interior_mask(eng)
+    rng = np.random.default_rng(11)
+    w = rng.normal(size=U0.size) * mask
+    dt = 1e-3
+    run(eng, U0, [dt])
+    gI, gkn = eng.adj_backward(w)
+    D = gpu_to_blocks(eng, U0)
+    W = gpu_to_blocks(eng, w)
+    orc = oracle.Oracle(deck.blocks, kn=0.01, limiter=limiter, scenario="tempjump")
+    fmap = ar.fill_transpose_map(deck.blocks)
+    geoms = [deck.geom(k) for k in range(deck.nblocks)]
+    gD, gk = ar.step_adjoint(orc, deck.blocks, geoms, lim, D, W, dt, 0.01, 0.0, fmap)
+    gD = ar.fill_T(gD, fmap)
+    G = gpu_to_blocks(eng, gI)
+    scale = max(np.abs(a).max() for a in gD)
+    worst = (0.0, None)
+    for k, (a, b) in enumerate(zip(G, gD)):
+        d = np.abs(a - b)
+        j = np.unravel_index(np.argmax(d), d.shape)
+        if d[j] > worst[0]:
+            worst = (float(d[j]), (k, j))
+    assert worst[0] <= 1e-9 * scale, (worst, scale)
+    assert abs(gkn - gk) <= 1e-9 * abs(gk), (gkn, gk)
+
+
+def test_zero_step_adjoint_is_fill_transpose(mbm):
+    """<fill^T g, u> == <g, fill(u)> with the GPU fill and its transpose."""
+    for name in ("box_small", "lshape_small", "periodic", "mixed_bc"):
+        deck = mbm.read_deck(deck_path(name))
+        eng = mbm.Engine(deck.blocks, kn=0.01)
+        eng.init()
+        rng = np.random.default_rng(5)
+        mask = interior_mask(eng)
+        u = rng.normal(size=mask.size) * mask
+        zero = np.zeros_like(u)
+        f_u = run(eng, u, [])
+        f_0 = run(eng, zero, [])            # Dirichlet ghosts are constants: use differences
+        g = rng.normal(size=mask.size)
+        for (off, plane, pitch), b in zip(eng.layer_index(), eng.blocks):
+            ly, lx = b["ny"] + 6, b["nx"] + 6
+            for c in range(9):
+                g[off + c * plane: off + c * plane + ly * pitch].reshape(ly, pitch)[:, lx:] = 0
+        run(eng, u, [])
+        gT, _ = eng.adj_backward(g)
+        lhs = gT @ u
+        rhs = g @ (f_u - f_0)
+        assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(rhs)), (name, lhs, rhs)
+
+
+@pytest.mark.parametrize("limiter", ["average", "none"])
 def test_one_step_adjoint_vs_fd(mbm, limiter):
     deck = mbm.read_deck(deck_path("box_small"))
     eng = mbm.Engine(deck.blocks, kn=0.01, limiter=limiter, scenario="tempjump")
@@ -84,16 +155,17 @@ def test_one_step_adjoint_vs_fd(mbm, limiter):
         h = 1e-6
         fd = (w @ run(eng, U0 + h * v, dts) - w @ run(eng, U0 - h * v, dts)) / (2 * h)
         assert abs(gI @ v - fd) <= 1e-6 * abs(fd), (trial, gI @ v, fd)
-    kn, h = 0.01, 1e-7
+    kn, h = 0.01, 1e-5
     fdk = (w @ run(eng, U0, dts, kn=kn * (1 + h)) - w @ run(eng, U0, dts, kn=kn * (1 - h))) / (2 * h * kn)
     assert abs(gkn - fdk) <= 1e-6 * abs(fdk), (gkn, fdk)
 
 
-def test_one_step_adjoint_vs_oracle_step(mbm):
+@pytest.mark.parametrize("limiter", ["average", "none"])
+def test_one_step_adjoint_vs_oracle_step(mbm, limiter):
     """Dot product with central differences of the CPU oracle's step (the
     reference algorithm), same state, same dt."""
     deck = mbm.read_deck(deck_path("lshape_small"))
-    eng = mbm.Engine(deck.blocks, kn=deck.kn, limiter=deck.limiter, scenario="tempjump")
+    eng = mbm.Engine(deck.blocks, kn=deck.kn, limiter=limiter, scenario="tempjump")
     eng.init()
     U0 = generic_state(mbm, eng, deck, seed=3)
     mask = interior_mask(eng)
@@ -102,7 +174,7 @@ def test_one_step_adjoint_vs_oracle_step(mbm):
     dt = 1e-3
     run(eng, U0, [dt])
     gI, _ = eng.adj_backward(w)
-    orc = oracle.Oracle(deck.blocks, kn=deck.kn, limiter=deck.limiter, scenario="tempjump")
+    orc = oracle.Oracle(deck.blocks, kn=deck.kn, limiter=limiter, scenario="tempjump")
     layout = eng.layer_index()
 
     def to_blocks(U):
@@ -135,9 +207,12 @@ def test_one_step_adjoint_vs_oracle_step(mbm):
         assert abs(gI @ v - fd) <= 1e-6 * abs(fd), (trial, gI @ v, fd)
 
 
-def test_multi_step_adjoint_with_checkpoints_vs_fd(mbm):
+@pytest.mark.parametrize("limiter", ["harmonic", "average"])
+def test_multi_step_adjoint_with_checkpoints_vs_fd(mbm, limiter):
+    """Checkpoint stride does not change the gradient (bitwise); for the
+    smooth limiter the gradient matches central differences of the run."""
     deck = mbm.read_deck(deck_path("small_kn01"))
-    eng = mbm.Engine(deck.blocks, kn=deck.kn, limiter="harmonic", scenario="tempjump")
+    eng = mbm.Engine(deck.blocks, kn=deck.kn, limiter=limiter, scenario="tempjump")
     eng.init()
     U0 = generic_state(mbm, eng, deck, seed=5)
     eng.layer_set(0, U0)
@@ -152,12 +227,14 @@ def test_multi_step_adjoint_with_checkpoints_vs_fd(mbm):
     run(eng, U0, dts, stride=23)
     g23, gkn23 = eng.adj_backward(w)
     assert np.array_equal(g5, g23) and gkn5 == gkn23      # checkpointing changes nothing
+    if limiter == "harmonic":
+        return              # kinks: finite differences measure secants here
     for trial in range(2):
         v = rng.normal(size=U0.size) * mask * (np.abs(U0) + 1e-2)
         h = 1e-6
         fd = (w @ run(eng, U0 + h * v, dts) - w @ run(eng, U0 - h * v, dts)) / (2 * h)
         assert abs(g5 @ v - fd) <= 1e-5 * abs(fd), (trial, g5 @ v, fd)
-    kn, h = deck.kn, 1e-7
+    kn, h = deck.kn, 1e-5
     fdk = (w @ run(eng, U0, dts, kn=kn * (1 + h)) - w @ run(eng, U0, dts, kn=kn * (1 - h))) / (2 * h * kn)
     assert abs(gkn5 - fdk) <= 1e-5 * abs(fdk), (gkn5, fdk)
 
@@ -165,7 +242,8 @@ def test_multi_step_adjoint_with_checkpoints_vs_fd(mbm):
 @pytest.mark.skipif(not HAVE_TORCH, reason="torch with a GPU is required")
 def test_torch_autograd_temperature_inversion(mbm):
     from multiblockmomentsfv_amd.adjoint import TemperatureInversion
-    prob = TemperatureInversion(deck_path("small_kn01"), kn=0.01, generic=1.0, seed=1)
+    prob = TemperatureInversion(deck_path("small_kn01"), kn=0.01, generic=1.0, seed=1,
+                                limiter="average")
     prob.record_schedule(tend=0.05, maxsteps=40)
     th0 = [t.clone().requires_grad_(True) for t in prob.theta0]
     kn = torch.tensor(0.01, dtype=torch.float64, requires_grad=True)
@@ -182,7 +260,7 @@ def test_torch_autograd_temperature_inversion(mbm):
         fd = float((jp - jm) / (2 * h))
         ad = float(sum((g.grad * d).sum() for g, d in zip(th0, dirs)))
         assert abs(ad - fd) <= 1e-5 * abs(fd), (trial, ad, fd)
-    h = 1e-7
+    h = 1e-5
     with torch.no_grad():
         jp = prob.objective(prob.theta0, torch.tensor(0.01 * (1 + h), dtype=torch.float64))
         jm = prob.objective(prob.theta0, torch.tensor(0.01 * (1 - h), dtype=torch.float64))

@@ -1,0 +1,63 @@
+"""Batched decks (no reference counterpart): every deck of a batch must
+advance exactly as it does alone through the plain engine (same kernels,
+same arithmetic: bit-identical), and as the oracle (1e-12)."""
+import numpy as np
+import pytest
+
+from conftest import REL_LINF_TOL, rel_linf
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def small_sweep(mbm, n=6, seed=3):
+    return mbm.sweep_decks(n=n, seed=seed, sides=(8, 12, 16, 24, 32))
+
+
+def test_batch_decks_equal_single_runs(mbm):
+    decks = small_sweep(mbm)
+    bat = mbm.Batch(decks)
+    bat.init()
+    ns, nr = bat.run(maxsteps=25)
+    assert (ns == 25).all()
+    for i, d in enumerate(decks):
+        eng = mbm.Engine(d["blocks"], kn=d["kn"])
+        eng.init()
+        t, n, rej, _ = eng.steps(0.0, 1e30, d["cfl"], maxsteps=25)
+        assert n == 25 and rej == nr[i]
+        assert t == bat.t[i]
+        for g, r in zip(bat.download(i), eng.download(0)):
+            assert g.tobytes() == r.tobytes(), f"deck {i}"
+
+
+def test_batch_against_oracle_and_second_run(mbm):
+    decks = small_sweep(mbm, n=4, seed=9)
+    bat = mbm.Batch(decks)
+    bat.init()
+    bat.run(maxsteps=10)
+    bat.run(maxsteps=7)          # second call: dt restarts from DT_INITIAL, DATA carries over
+    for i, d in enumerate(decks):
+        orc = oracle.Oracle(d["blocks"], kn=d["kn"])
+        orc.init()
+        orc.steps(1e30, d["cfl"], maxsteps=10)
+        orc.steps(1e30, d["cfl"], maxsteps=7)
+        assert rel_linf(bat.download(i), orc.get(0)).max() <= REL_LINF_TOL
+
+
+def test_batch_tend_and_autotune(mbm):
+    decks = small_sweep(mbm, n=5, seed=1)
+    for i, d in enumerate(decks):
+        d["tend"] = 0.01 * (i + 1)
+    bat = mbm.Batch(decks)
+    bat.init()
+    best, ms = bat.autotune(rows=(4, 8, 16), reps=2)
+    assert best in (4, 8, 16) and ms > 0
+    ns, nr = bat.run(maxsteps=0)
+    for i, d in enumerate(decks):
+        assert bat.t[i] == pytest.approx(d["tend"], abs=1e-15)
+        eng = mbm.Engine(d["blocks"], kn=d["kn"])
+        eng.init()
+        t, n, rej, _ = eng.steps(0.0, d["tend"], d["cfl"])
+        assert n == ns[i]
+        assert rel_linf(bat.download(i), eng.download(0)).max() == 0.0

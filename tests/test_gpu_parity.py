@@ -145,9 +145,9 @@ def test_every_limiter_on_a_smooth_deck(mbm, limiter):
     orc = oracle.Oracle(deck.blocks, kn=deck.kn, limiter=limiter, scenario="tempjump")
     orc.set_tempjump(*params)
     orc.init()
-    _, n, _, log = eng.steps(0.0, deck.tout[0], deck.cfl, maxsteps=30)
-    n2, log2 = orc.steps(deck.tout[0], deck.cfl, maxsteps=30)
-    assert n == n2 == 30
+    _, n, _, log = eng.steps(0.0, deck.tout[0], deck.cfl, maxsteps=20)
+    n2, log2 = orc.steps(deck.tout[0], deck.cfl, maxsteps=20)
+    assert n == n2 == 20
     assert np.allclose(log[:, :3], log2[:, :3], rtol=REL_LINF_TOL, atol=0)
     err = rel_linf(eng.download(0), orc.get(0))
     assert err.max() <= REL_LINF_TOL, err
