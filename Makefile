@@ -15,7 +15,10 @@ ARCH    ?= gfx950
 PKG     := multiblockmomentsfv_amd
 GIT_COMMIT := $(shell git rev-parse --short HEAD 2>/dev/null || echo unknown)
 
-HIPFLAGS := --offload-arch=$(ARCH) -O3 -std=c++17 -fPIC -Wall -Iinclude \
+# -ffp-contract=on: a*b+c is fused only within one source expression, so every
+# inlined copy of a formula rounds the same way and results do not depend on
+# the tiling (the default 'fast' lets the backend fuse per call site).
+HIPFLAGS := --offload-arch=$(ARCH) -O3 -std=c++17 -fPIC -Wall -Iinclude -ffp-contract=on \
             -Xclang -target-feature -Xclang -packed-fp32-ops \
             -Wno-unused-command-line-argument
 CFLAGS   := -O2 -fPIC -Wall -Iinclude -DMBM_GIT_COMMIT=\"$(GIT_COMMIT)\"

@@ -80,6 +80,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise MbmError(f"{LIB_PATH} is not built: run `make` at the repository root")
+    # PyTorch ships its own HIP runtime whose soname matches the one the
+    # engine links against. Loading torch first makes the engine bind to that
+    # same runtime, so both share one device context and torch tensors can be
+    # handed to the engine; loaded the other way round, two runtimes end up in
+    # the process and torch cannot open the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     p, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     sigs = {

@@ -31,7 +31,9 @@ def generic_blocks(deck, seed):
         bx, by, dx, dy = deck.geom(k)
         X, Y = np.meshgrid(bx + dx * np.arange(b["nx"] + 6), by + dy * np.arange(b["ny"] + 6))
         dr, v1, v2, d11, d22, t12, q1, q2 = smooth_fields(X, Y, rng, 1.0)
-        th = 1.0 + 0.5 * (1 - np.tanh((np.hypot(X, Y) - 0.4) / 0.1))
+        # off-centre: a profile symmetric about the block interfaces x=0, y=0 makes
+        # the theta jump across them vanish to round-off, a limiter kink
+        th = 1.0 + 0.5 * (1 - np.tanh((np.hypot(X - 0.137, Y + 0.071) - 0.4) / 0.1))
         P = np.stack([1 + dr, v1, v2, th, th + d11, t12, th + d22, q1, q2], -1)
         U = np.empty_like(P)
         for j in range(P.shape[0]):

