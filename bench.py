@@ -33,7 +33,7 @@ BYTES_PER_CELL_UPDATE = 360.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_square4_1024_traffic.json")
 
 
-def cpu_baseline(deck_path, cells, steps=2):
+def cpu_baseline(deck_path, cells, steps=12):
     """Times the reference build (or the oracle port) on a few steps."""
     import oracle
     threads = min(4, os.cpu_count() or 1)
@@ -134,7 +134,7 @@ def main():
                         "Kn=0.01, CFL=0.9, harmonic, Riemann2D initial state, adaptive dt",
             "cells": cells, "blocks": deck.nblocks, "kn": deck.kn, "r13": deck.r13,
             "cfl": deck.cfl, "limiter": deck.limiter, "t_start": round(t - 0, 6),
-            "rejected_steps": rej,
+            "rejected_steps": rej, "tile_rows": eng.tile_rows(),
         },
         "roofline": {
             "bound": "hbm",

@@ -81,7 +81,10 @@ int mbm_create(const mbm_block_desc *blocks, int nblocks, const mbm_params *p,
 int mbm_destroy(mbm_ctx *c);                                  /* Main.c:158-161 */
 int mbm_set_stream(mbm_ctx *c, void *stream);
 int mbm_set_params(mbm_ctx *c, const mbm_params *p);          /* CLI overrides, Main.c:97-106 */
-int mbm_set_tile_rows(mbm_ctx *c, int rows);                  /* tile height of the stage kernel */
+/* Tile height (rows per wavefront) of the stage kernel; 0 (the default)
+ * chooses it from the deck and the device's compute-unit count. */
+int mbm_set_tile_rows(mbm_ctx *c, int rows);
+int mbm_tile_rows(const mbm_ctx *c);                          /* height in use */
 
 /* Initial state on interior cells then the ghost fill (Main.c:124-138). */
 int mbm_init(mbm_ctx *c);

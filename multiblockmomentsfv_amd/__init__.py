@@ -99,6 +99,7 @@ def lib():
         "mbm_set_stream": (i, [p, p]),
         "mbm_set_params": (i, [p, ctypes.POINTER(Params)]),
         "mbm_set_tile_rows": (i, [p, i]),
+        "mbm_tile_rows": (i, [p]),
         "mbm_init": (i, [p]),
         "mbm_upload": (i, [p, i, i, p]),
         "mbm_download": (i, [p, i, i, p]),
@@ -157,7 +158,7 @@ def lib():
 def exported_symbols():
     """Names the C headers declare (checked against the library by the tests)."""
     return [n for n in ("mbm_last_error", "mbm_create", "mbm_destroy", "mbm_set_stream",
-                        "mbm_set_params", "mbm_set_tile_rows", "mbm_init", "mbm_upload",
+                        "mbm_set_params", "mbm_set_tile_rows", "mbm_tile_rows", "mbm_init", "mbm_upload",
                         "mbm_download", "mbm_layer_ptr", "mbm_set_boundary",
                         "mbm_time_update", "mbm_commit", "mbm_steps", "mbm_output_values",
                         "mbm_interior_cells", "mbm_block_dims", "mbm_enable_timing",
@@ -285,7 +286,11 @@ class Engine:
         return (self.blocks[k]["ny"] + 6, self.blocks[k]["nx"] + 6, NV)
 
     def set_tile_rows(self, rows):
+        """Rows per wavefront of the stage kernel; 0 = automatic."""
         _check(lib().mbm_set_tile_rows(self.h, int(rows)), "mbm_set_tile_rows")
+
+    def tile_rows(self):
+        return lib().mbm_tile_rows(self.h)
 
     def init(self):
         _check(lib().mbm_init(self.h), "mbm_init")

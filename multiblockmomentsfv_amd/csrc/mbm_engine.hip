@@ -84,22 +84,54 @@ MBM_DEV void load9(const double *__restrict__ p, long long plane, double *U)
   for (int m = 0; m < NV; m++) U[m] = p[m * plane];
 }
 
+/* Raw buffer access to one block of a layer (gfx9 buffer resource: base,
+ * stride 0, byte extent, DATA_FORMAT=32 in the last word). The component
+ * offset m*plane*8 goes in the scalar offset and the lane's byte offset in one
+ * 32-bit VGPR, so nine loads share one address register; accesses past the
+ * extent read 0 / are dropped by the hardware. */
+MBM_DEV __amdgpu_buffer_rsrc_t block_rsrc(const double *base, long long plane)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0,
+                                           (int)(NV * plane * 8), 0x00020000);
+}
+
+MBM_DEV void load9b(__amdgpu_buffer_rsrc_t r, int pbytes, unsigned boff, double *U)
+{
+#pragma unroll
+  for (int m = 0; m < NV; m++)
+    U[m] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, boff, m * pbytes, 0));
+}
+
+MBM_DEV void store1b(__amdgpu_buffer_rsrc_t r, int soff, unsigned boff, double v)
+{
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, boff, soff, 0);
+}
+
 } // namespace mbm
 
 using namespace mbm;
 
+/* waves per SIMD the stage kernel is compiled for: 1 gives each wave the
+ * whole 512-entry register file (256 VGPRs + AGPRs), so nothing spills; the
+ * row prefetch and the nine independent components hide the latencies */
+#ifndef MBM_STAGE_WAVES
+#define MBM_STAGE_WAVES 1
+#endif
+
 template <int STAGE, int LIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) stage_kernel(const StageArgs A)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MBM_STAGE_WAVES, MBM_STAGE_WAVES)))
+stage_kernel(const StageArgs A)
 {
   const int lane = threadIdx.x & 63;
-  const int tix = blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6);
+  /* wave-uniform: tile, block and deck data are loaded into scalar registers */
+  const int tix = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6));
   if (tix >= A.ntiles) return;
   const TileDev T = A.tiles[tix];
   const BlockDev B = A.blocks[T.blk];
   const int lx = B.lx, ly = B.ly;
   const int a = T.a0 + lane;
   const int ac = a < lx ? a : lx - 1;
-  const long long pl = B.plane;
   const double *in = A.in, *in0 = A.in0;
   double *out = A.out;
   double dt = A.dt, dtkn = A.dtkn, force = A.force;
@@ -115,9 +147,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     in0 = data;
     out = STAGE == 1 ? A.tmp : newb;
   }
-  const double *__restrict__ inb = in + B.off + ac;
-  const double *__restrict__ in0b = (STAGE == 2) ? in0 + B.off + ac : nullptr;
-  double *__restrict__ outb = out + B.off + a;
+  /* block bases (uniform) and per-lane byte offsets within a component plane;
+   * the host guarantees 8*plane < 2^31 */
+  const __amdgpu_buffer_rsrc_t rin = block_rsrc(in + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rin0 = block_rsrc(in0 + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rout = block_rsrc(out + B.off, B.plane);
+  const int pb = (int)(8 * B.plane);
+  const unsigned rowb = 8u * (unsigned)B.pitch;
+  const unsigned cin = 8u * (unsigned)ac, cout = 8u * (unsigned)a;
   const double dtdx = dt / B.dx, dtdy = dt / B.dy;
 
   const bool col_ring3 = (a == 0 || a >= lx - 1);
@@ -137,19 +174,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   double Pa[NV], Pb[NV], Pc[NV], Pn[NV], In1[NV], acc[NV], Tup[NV];
   {
     int rr = row_of(r0 - 2);
-    load9(inb + (long long)rr * B.pitch, pl, U);
+    load9b(rin, pb, cin + rowb * rr, U);
     project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
     rr = row_of(r0 - 1);
-    load9(inb + (long long)rr * B.pitch, pl, U);
+    load9b(rin, pb, cin + rowb * rr, U);
     project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp);
     rr = row_of(r0);
-    load9(inb + (long long)rr * B.pitch, pl, U);
-    if constexpr (STAGE == 2) load9(in0b + (long long)rr * B.pitch, pl, U0);
+    load9b(rin, pb, cin + rowb * rr, U);
+    if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb * rr, U0);
     project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp);
     if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
     rr = row_of(r0 + 1);
-    load9(inb + (long long)rr * B.pitch, pl, U);
-    if constexpr (STAGE == 2) load9(in0b + (long long)rr * B.pitch, pl, U0);
+    load9b(rin, pb, cin + rowb * rr, U);
+    if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb * rr, U0);
     project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, In1, cp);
     if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
 
@@ -171,9 +208,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   /* ---- march ---- */
   for (int r = r0; r <= T.r1; r++) {
     /* prefetch row r+2 (r+2 <= ly-1 since r <= ly-3) */
-    const long long roff = (long long)(r + 2) * B.pitch;
-    load9(inb + roff, pl, U);
-    if constexpr (STAGE == 2) load9(in0b + roff, pl, U0);
+    const unsigned roff = cin + rowb * (unsigned)(r + 2);
+    load9b(rin, pb, roff, U);
+    if constexpr (STAGE == 2) load9b(rin0, pb, roff, U0);
 
     /* x-faces of row r */
     {
@@ -212,11 +249,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       face<1>(Tup, Bn, dtdy, G, s);
       if (col_face) fold_max(mxy, s);
       const bool st = st_col && r >= MBC && r <= ly - 1 - MBC;
-      double *o = outb + (long long)r * B.pitch;
+      const unsigned ob = cout + rowb * (unsigned)r;
 #pragma unroll
       for (int n = 0; n < NV; n++) {
         const double v = acc[n] - G[n];
-        if (st) o[n * pl] = (STAGE == 2) ? 0.5 * v : v;
+        if (st) store1b(rout, n * pb, ob, (STAGE == 2) ? 0.5 * v : v);
         acc[n] = In1[n] + G[n];
         In1[n] = I2[n];
         Tup[n] = Tn[n];
@@ -507,8 +544,10 @@ int validate(const mbm_block_desc *blocks, int nb, const mbm_params *p)
     const mbm_block_desc &B = blocks[k];
     if (B.nx < MBC || B.ny < MBC)
       return set_err(MBM_E_ARG, "block %d: grid %dx%d smaller than 3x3", k + 1, B.nx, B.ny);
-    if ((long long)(B.nx + 6) * (B.ny + 6) * NV >= (1ll << 31))
-      return set_err(MBM_E_ARG, "block %d too large", k + 1);
+    /* the stage kernel addresses a block with 32-bit byte offsets:
+     * 72 * (padded plane) < 2^31 */
+    if ((long long)(B.nx + 6 + 7) * (B.ny + 6) * 8 * NV + 256 >= (1ll << 31))
+      return set_err(MBM_E_ARG, "block %d too large (%dx%d)", k + 1, B.nx, B.ny);
     if (!(B.xmax > B.xmin) || !(B.ymax > B.ymin))
       return set_err(MBM_E_ARG, "block %d has an empty extent", k + 1);
     for (int s = 0; s < 4; s++) {
@@ -587,9 +626,43 @@ std::vector<TileDev> build_tiles(const std::vector<mbm_block_desc> &D, int rows)
   return t;
 }
 
+/* Tile height for a deck: every tile is one wavefront and all tiles cost the
+ * same, so the launch takes ceil(ntiles / slots) rounds of one tile time
+ * (rows + a warm-up of about 4 rows); slots = resident stage waves. */
+int auto_tile_rows(const std::vector<mbm_block_desc> &D, int slots)
+{
+  int best = 32;
+  double bestc = 1e300;
+  int maxrows = 0;
+  for (const auto &b : D) maxrows = std::max(maxrows, b.ny + 3);
+  for (int R = 4; R <= 512; R++) {
+    long long nt = 0;
+    for (const auto &b : D) {
+      const int lx = b.nx + 2 * MBC;
+      int strips = 0;
+      for (int a0 = 0;; a0 += STRIP) {
+        strips++;
+        if (a0 + 61 >= lx - 3) break;
+      }
+      nt += (long long)strips * ((b.ny + 3 + R - 1) / R);
+    }
+    const long long rounds = (nt + slots - 1) / slots;
+    const double cost = (double)rounds * (std::min(R, maxrows) + 4);
+    if (cost < bestc - 1e-9) {
+      bestc = cost;
+      best = R;
+    }
+    if (R >= maxrows) break;
+  }
+  return best;
+}
+
 int upload_tiles(mbm_ctx *c)
 {
-  std::vector<TileDev> t = build_tiles(c->desc, c->tile_rows);
+  const int rows = c->tile_rows > 0 ? c->tile_rows
+                                    : auto_tile_rows(c->desc, c->cus * 4 * MBM_STAGE_WAVES);
+  c->rows_used = rows;
+  std::vector<TileDev> t = build_tiles(c->desc, rows);
   if (c->dtiles) HIPCHK(hipFree(c->dtiles));
   c->dtiles = nullptr;
   HIPCHK(hipMalloc(&c->dtiles, t.size() * sizeof(TileDev)));
@@ -676,7 +749,14 @@ mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_f
   c->nb = nblocks;
   c->desc.assign(blocks, blocks + nblocks);
   c->par = deck_par[0];
-  c->tile_rows = 32;
+  c->tile_rows = 0;                      /* automatic */
+  c->cus = 256;
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      c->cus = cus;
+  }
   c->ndecks = ndecks;
   c->deck_first.assign(deck_first, deck_first + ndecks + 1);
   c->deck_par.assign(deck_par, deck_par + ndecks);
@@ -914,10 +994,12 @@ extern "C" int mbm_set_params(mbm_ctx *c, const mbm_params *p)
   return upload_params(c);
 }
 
+extern "C" int mbm_tile_rows(const mbm_ctx *c) { return c ? c->rows_used : 0; }
+
 extern "C" int mbm_set_tile_rows(mbm_ctx *c, int rows)
 {
   if (!c) return set_err(MBM_E_ARG, "NULL context");
-  if (rows < 1 || rows > 4096) return set_err(MBM_E_ARG, "tile rows %d outside [1,4096]", rows);
+  if (rows < 0 || rows > 4096) return set_err(MBM_E_ARG, "tile rows %d outside [0,4096]", rows);
   c->tile_rows = rows;
   HIPCHK(hipStreamSynchronize(c->stream));
   release_graphs(c);

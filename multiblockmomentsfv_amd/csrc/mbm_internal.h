@@ -106,7 +106,9 @@ struct mbm_ctx {
   BlockDev *dblk;
   TileDev *dtiles;
   int ntiles;
-  int tile_rows;
+  int tile_rows;             /* requested tile height, 0 = automatic */
+  int rows_used;             /* tile height of the current tile list */
+  int cus;                   /* compute units of the device */
   GhostDev *dghost;
   int nghost;
   unsigned long long *dslots; /* 2 stages x nb x 3 */
