@@ -22,9 +22,9 @@
  *   SetInitial(TempJump)                       -> build-defined scenario with
  *       no reference counterpart; pinned by closed-form checks of the initial
  *       state (test_oracle_math.py).
- *   tangent (orc_tangent_*)                    -> pinned against central finite
- *       differences of the oracle itself and a complex-free dual-number check
- *       of the closed-form derivative of single cell functions.
+ *   The oracle has no derivatives: the engine's adjoint is checked against
+ *   central differences of orc_time_update / orc_update_block
+ *   (test_adjoint_local.py, test_adjoint_step_ref.py).
  */
 #ifndef MBM_ORACLE_H
 #define MBM_ORACLE_H
