@@ -1,20 +1,25 @@
 """Benchmark of the explicit update on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--deck PATH] [--no-cpu]
+                    [--no-batched] [--no-adjoint]
 
 Workload: square4 topology with every block 1024x1024 (4 194 304 cells),
 Reg13, Kn=0.01, CFL=0.9, harmonic limiter, Riemann2D initial state, FP64,
 adaptive dt (multiblockmomentsfv_amd/decks/square4_1024.txt).  A step is one
-accepted Heun step (TimeUpdate + commit) driven by the Steps controller; the
-state stays in HBM, the host reads one 8-byte CFL per step.
+Heun step attempt (TimeUpdate, then accept + commit or reject) driven by the
+Steps controller on the device; the state stays in HBM.
 
-Prints one JSON line: value = cell-updates/s (interior cells x accepted
-steps / wall time of the K timed steps).  roofline: the stage kernel's
-algorithmic HBM bytes (stage 1 reads U^n and writes U*, stage 2 reads U*,
-U^n and writes U^n+1: 360 B per cell-update, 180 B per cell per launch on
-average) over its mean launch time measured with HIP events on the engine's
-stream.  cpu_baseline: the reference itself (oracle/_ref) on the host cores,
-a bounded sample of the same deck.
+Prints one JSON line: value = cell-updates/s (interior cells x attempts /
+wall time of the K timed steps).  roofline: the stage kernel's algorithmic
+HBM bytes (stage 1 reads U^n and writes U*, stage 2 reads U*, U^n and writes
+U^n+1: 360 B per cell-update, 180 B per cell per launch on average) over its
+mean launch time measured with HIP events on the engine's stream.
+batched: 256 seeded decks of uneven sizes and Kn advanced together, against
+the same decks run one after another.  adjoint: cost of the gradient of a
+temperature-matching objective over ~2000 steps on 4x512^2 relative to the
+plain forward run, and its agreement with central differences.
+cpu_baseline: the reference itself (oracle/_ref) on the host cores, a bounded
+sample of the same deck.
 """
 import argparse
 import json
@@ -23,10 +28,13 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 DECK = os.path.join(ROOT, "multiblockmomentsfv_amd", "decks", "square4_1024.txt")
+ADJ_DECK = os.path.join(ROOT, "multiblockmomentsfv_amd", "decks", "square4_512.txt")
 METRIC = "FP64 cell-updates/s + % HBM roofline; batched & adjoint vs plain step"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 360.0
@@ -63,6 +71,110 @@ def cpu_baseline(deck_path, cells, steps=12):
             "kind": "port", "sample": f"{n} Heun steps of the same deck (C oracle)"}
 
 
+def batched_leg(ndecks, steps, plain_steps, seed=0):
+    """256 seeded square4-topology decks (block sides 64..512, Kn log-uniform
+    in [1e-3, 1e-1]) advanced together for `steps` accepted steps each, against
+    the same decks run one after another through the plain single-deck path."""
+    import torch
+    import multiblockmomentsfv_amd as m
+    decks = m.sweep_decks(ndecks, seed=seed)
+    bat = m.Batch(decks)
+    bat.init()
+    best, best_ms = bat.autotune(rows=(8, 16, 24, 32, 48, 64, 96), reps=2)
+    bat.init()
+    cells = np.array([sum(b["nx"] * b["ny"] for b in d["blocks"]) for d in decks], dtype=np.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ns, nr = bat.run(maxsteps=steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    bat.close()
+    batched = float((cells * (ns + nr)).sum() / wall)
+
+    plain_cu, plain_s = 0.0, 0.0
+    for d, c in zip(decks, cells):
+        eng = m.Engine(d["blocks"], kn=d["kn"])
+        eng.init()
+        t, _, _, _ = eng.steps(0.0, 1e30, d["cfl"], maxsteps=3, reccap=0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, n, rej, _ = eng.steps(t, 1e30, d["cfl"], maxsteps=plain_steps, reccap=0, dt=eng.next_dt)
+        torch.cuda.synchronize()
+        plain_s += time.perf_counter() - t0
+        plain_cu += c * (n + rej)
+        eng.close()
+    plain = plain_cu / plain_s
+    return {"decks": ndecks, "cells": int(cells.sum()), "steps_per_deck": steps,
+            "attempts": int((ns + nr).sum()), "value": batched, "unit": "cell-updates/s",
+            "seconds": wall, "tile_rows": best,
+            "plain_per_deck": plain, "plain_steps_per_deck": plain_steps,
+            "ratio_vs_plain": batched / plain,
+            "config": f"seed {seed}: square4 topology, block sides from {{64,...,512}}, "
+                      "Kn log-uniform [1e-3,1e-1], CFL 0.95, harmonic, Riemann2D"}
+
+
+def adjoint_leg(deck_path, fd_dirs, tend=0.3):
+    """Gradient of J = sum (theta(t_end) - theta_target)^2 with respect to the
+    initial temperature field and Kn on 4x512^2, against central differences."""
+    import torch
+    import multiblockmomentsfv_amd as m
+    from multiblockmomentsfv_amd.adjoint import TemperatureInversion
+    # generic=0.1 adds smooth fields of 1-5 % to every primitive: the plain jump
+    # starts at rest, where every slope is exactly zero and the limiters sit on
+    # their kinks (finite differences there measure the limiter's nonlinear
+    # response, not a derivative)
+    prob = TemperatureInversion(deck_path, kn=0.01, seed=0, generic=0.1)
+    nsteps, _ = prob.record_schedule(tend=tend)
+    # plain forward of the same number of steps (graph-replayed controller)
+    eng = prob.engine
+    eng.layer_set(m.L_DATA, prob.initial_state(prob.theta0))
+    eng.set_boundary(m.L_DATA, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, n_plain, rej_plain, _ = eng.steps(0.0, 1e30, prob.deck.cfl, maxsteps=nsteps, reccap=0)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    th0 = [t.clone().requires_grad_(True) for t in prob.theta0]
+    kn = torch.tensor(prob.kn, dtype=torch.float64, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = prob.objective(th0, kn)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    errs = []
+    h = 1e-6
+    for _ in range(fd_dirs):
+        dirs = [torch.tensor(rng.normal(size=t.shape), dtype=torch.float64, device=t.device)
+                for t in prob.theta0]
+        with torch.no_grad():
+            jp = float(prob.objective([t + h * d for t, d in zip(prob.theta0, dirs)], kn.detach()))
+            jm = float(prob.objective([t - h * d for t, d in zip(prob.theta0, dirs)], kn.detach()))
+        fd = (jp - jm) / (2 * h)
+        ad = float(sum((g.grad * d).sum() for g, d in zip(th0, dirs)))
+        errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
+    with torch.no_grad():
+        hk = 1e-5
+        jp = float(prob.objective(prob.theta0, torch.tensor(prob.kn * (1 + hk), dtype=torch.float64)))
+        jm = float(prob.objective(prob.theta0, torch.tensor(prob.kn * (1 - hk), dtype=torch.float64)))
+    fdk = (jp - jm) / (2 * hk * prob.kn)
+    return {"cells": prob.deck.interior_cells(), "steps": nsteps,
+            "plain_forward_s": t_plain, "forward_with_checkpoints_s": t_fwd, "backward_s": t_bwd,
+            "gradient_over_forward": (t_fwd + t_bwd) / t_plain, "backward_over_forward": t_bwd / t_plain,
+            "fd_directions": fd_dirs, "fd_rel_err_max": max(errs) if errs else None,
+            "fd_rel_err_median": float(np.median(errs)) if errs else None,
+            "dJ_dKn": float(kn.grad), "dJ_dKn_fd": fdk,
+            "dJ_dKn_rel_err": abs(float(kn.grad) - fdk) / abs(fdk),
+            "config": "square4 topology 4x512^2, Reg13, Kn 0.01, CFL 0.9, harmonic, "
+                      "tanh temperature jump (5 inside r<0.4, 1 outside, width 0.05) plus smooth "
+                      "fields of 1-5 % in every primitive, t_end 0.3; FD: white-noise "
+                      "directions on theta0, h=1e-6"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +184,12 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-decks", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=1000)
+    ap.add_argument("--plain-steps", type=int, default=50)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--fd-dirs", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -150,9 +268,13 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
     }
+    eng.close()
+    if not args.no_batched:
+        out["batched"] = batched_leg(args.batched_decks, args.batched_steps, args.plain_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(ADJ_DECK, args.fd_dirs)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.deck, cells)
-    eng.close()
     print(json.dumps(out))
 
 

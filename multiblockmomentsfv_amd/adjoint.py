@@ -88,9 +88,10 @@ class TemperatureInversion:
     def __init__(self, deck_path, kn=None, rho=1.0, seed=0, device="cuda", generic=0.0,
                  limiter=None):
         """generic > 0 adds smooth fields of that relative amplitude to every
-        primitive of the base state, so that no slope is exactly zero (at a
-        state at rest the limiters sit on their kinks and finite differences
-        measure secants, not derivatives)."""
+        primitive of the base state (theta0 included), so that no slope is
+        exactly zero (at a state at rest, or where the tanh profile is flat, the
+        limiters sit on their kinks and finite differences measure the
+        limiter's nonlinear response, not a derivative)."""
         self.deck = read_deck(deck_path)
         self.kn = self.deck.kn if kn is None else kn
         self.rho = rho
@@ -118,6 +119,10 @@ class TemperatureInversion:
             if generic > 0:
                 self.extra.append([torch.tensor(f, dtype=torch.float64, device=device)
                                    for f in smooth_fields(X, Y, rng, generic)])
+                # theta itself: the tanh profile is exactly flat away from the jump
+                dth = smooth_fields(X, Y, rng, generic)[3]
+                self.theta0[k] = self.theta0[k] + torch.tensor(dth, dtype=torch.float64,
+                                                               device=device)
             else:
                 self.extra.append(None)
         self.dts = None

@@ -146,7 +146,7 @@ MBM_DEV Damp damp3_dz(double i1, double i2)
 /* z = rho sqrt(theta) dt/Kn: adds to gQ and to gkn (d/dKn = -z/Kn) */
 MBM_DEV void z_vjp(const double *Q, double dtkn, double kn, double gz, double *gQ, double &gkn)
 {
-  const double sq = sqrt(Q[3]);
+  const double sq = sqrtp(Q[3]);
   gQ[0] += sq * dtkn * gz;
   gQ[3] += Q[0] * dtkn * 0.5 / sq * gz;
   gkn -= Q[0] * sq * dtkn / kn * gz;
@@ -303,7 +303,7 @@ MBM_DEV void hll_vjp(const double *WL, const double *WR, double dtdx, const doub
                      double *gWL, double *gWR)
 {
   const double k = 20.0 / 3.0;
-  const double cL = sqrt(k * WL[3]), cR = sqrt(k * WR[3]);
+  const double cL = sqrtp(k * WL[3]), cR = sqrtp(k * WR[3]);
   const double lamL = fmin(WL[1] - cL, 0.0);
   const double lamR = fmax(WR[1] + cR, 0.0);
   const double alph = lamR - lamL;

@@ -141,7 +141,7 @@ MBM_DEV double adj_cell_cell(const double *in, const double *in0, const double *
       adj_ld(in, pl, off, U);
       primitive(U, Q);
       const double dtkn = dt / kn;
-      const double z = Q[0] * sqrt(Q[3]) * dtkn;
+      const double z = Q[0] * sqrtp(Q[3]) * dtkn;
       double i1, i2;
       damp_inv(z, i1, i2);
       double gQ[9] = { 0 }, gz = 0.0, g1 = 0.0, g2 = 0.0;
@@ -174,7 +174,7 @@ MBM_DEV double adj_cell_cell(const double *in, const double *in0, const double *
         double U0[9], Q0[9], R0[9], gR0[9] = { 0 }, gQ0[9] = { 0 };
         adj_ld(in0, pl, off, U0);
         primitive(U0, Q0);
-        const double z0 = Q0[0] * sqrt(Q0[3]) * dtkn;
+        const double z0 = Q0[0] * sqrtp(Q0[3]) * dtkn;
         double j1, j2;
         damp_inv(z0, j1, j2);
         const Damp e0 = damp0(z0, j1, j2);

@@ -52,14 +52,15 @@ namespace mbm {
 
 /* wavefront-wide DPP shifts on the two 32-bit halves of a double:
  * wave_shr:1 (lane i <- lane i-1), wave_shl:1 (lane i <- lane i+1); the lane
- * without a source keeps its own value (never used by the stencil). */
+ * without a source reads 0 (bound_ctrl), a halo lane the stencil never uses.
+ * Without an 'old' operand no copy of the destination is needed first. */
 template <int CTRL>
 MBM_DEV double dpp_shift(double v)
 {
   const long long b = __double_as_longlong(v);
   const int lo = (int)b, hi = (int)(b >> 32);
-  const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
-  const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  const int l2 = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
+  const int h2 = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
   return __longlong_as_double(((long long)h2 << 32) | (unsigned int)l2);
 }
 MBM_DEV double from_left(double v) { return dpp_shift<0x138>(v); }
@@ -191,13 +192,28 @@ stage_kernel(const StageArgs A)
     if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
 
     double Tprev[NV], Bot[NV], G[NV], s;
+    /* slopes in groups of three, exactly as in the march (same rounding) */
 #pragma unroll
-    for (int n = 0; n < NV; n++) {
-      const double sp = limit<LIM>(Pb[n] - Pa[n], Pc[n] - Pb[n]);
-      Tprev[n] = Pb[n] + 0.5 * sp;
-      const double sc = limit<LIM>(Pc[n] - Pb[n], Pn[n] - Pc[n]);
-      Tup[n] = Pc[n] + 0.5 * sc;
-      Bot[n] = Pc[n] - 0.5 * sc;
+    for (int g = 0; g < NV; g += 3) {
+      double e1[3], e2[3], sp[3], sc[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        e1[k] = Pb[g + k] - Pa[g + k];
+        e2[k] = Pc[g + k] - Pb[g + k];
+      }
+      limit3<LIM>(e1, e2, sp);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        e1[k] = Pc[g + k] - Pb[g + k];
+        e2[k] = Pn[g + k] - Pc[g + k];
+      }
+      limit3<LIM>(e1, e2, sc);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        Tprev[g + k] = Pb[g + k] + 0.5 * sp[k];
+        Tup[g + k] = Pc[g + k] + 0.5 * sc[k];
+        Bot[g + k] = Pc[g + k] - 0.5 * sc[k];
+      }
     }
     face<1>(Tprev, Bot, dtdy, G, s);
     if (col_face && r0 - 1 >= 1 && r0 - 1 <= ly - 3) fold_max(mxy, s);
@@ -216,12 +232,19 @@ stage_kernel(const StageArgs A)
     {
       double L[NV], Rr[NV], F[NV], s;
 #pragma unroll
-      for (int n = 0; n < NV; n++) {
-        const double pm = from_left(Pc[n]);
-        const double pp = from_right(Pc[n]);
-        const double sx = limit<LIM>(Pc[n] - pm, pp - Pc[n]);
-        L[n] = Pc[n] + 0.5 * sx;
-        Rr[n] = from_right(Pc[n] - 0.5 * sx);
+      for (int g = 0; g < NV; g += 3) {
+        double e1[3], e2[3], sx[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          e1[k] = Pc[g + k] - from_left(Pc[g + k]);
+          e2[k] = from_right(Pc[g + k]) - Pc[g + k];
+        }
+        limit3<LIM>(e1, e2, sx);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          L[g + k] = Pc[g + k] + 0.5 * sx[k];
+          Rr[g + k] = from_right(Pc[g + k] - 0.5 * sx[k]);
+        }
       }
       face<0>(L, Rr, dtdx, F, s);
       if (xface_ok) fold_max(mxx, s);
@@ -241,19 +264,32 @@ stage_kernel(const StageArgs A)
     {
       double Tn[NV], Bn[NV], G[NV], s;
 #pragma unroll
-      for (int n = 0; n < NV; n++) {
-        const double sy = limit<LIM>(Pn[n] - Pc[n], P2[n] - Pn[n]);
-        Tn[n] = Pn[n] + 0.5 * sy;
-        Bn[n] = Pn[n] - 0.5 * sy;
+      for (int g = 0; g < NV; g += 3) {
+        double e1[3], e2[3], sy[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          e1[k] = Pn[g + k] - Pc[g + k];
+          e2[k] = P2[g + k] - Pn[g + k];
+        }
+        limit3<LIM>(e1, e2, sy);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          Tn[g + k] = Pn[g + k] + 0.5 * sy[k];
+          Bn[g + k] = Pn[g + k] - 0.5 * sy[k];
+        }
       }
       face<1>(Tup, Bn, dtdy, G, s);
       if (col_face) fold_max(mxy, s);
-      const bool st = st_col && r >= MBC && r <= ly - 1 - MBC;
-      const unsigned ob = cout + rowb * (unsigned)r;
+      if (st_col && r >= MBC && r <= ly - 1 - MBC) {
+        const unsigned ob = cout + rowb * (unsigned)r;
+#pragma unroll
+        for (int n = 0; n < NV; n++) {
+          const double v = acc[n] - G[n];
+          store1b(rout, n * pb, ob, (STAGE == 2) ? 0.5 * v : v);
+        }
+      }
 #pragma unroll
       for (int n = 0; n < NV; n++) {
-        const double v = acc[n] - G[n];
-        if (st) store1b(rout, n * pb, ob, (STAGE == 2) ? 0.5 * v : v);
         acc[n] = In1[n] + G[n];
         In1[n] = I2[n];
         Tup[n] = Tn[n];

@@ -47,6 +47,28 @@ MBM_DEV double rcp(double x)
 #endif
 }
 
+/* square root of a positive number: hardware rsq estimate, one Goldschmidt
+ * step and two Newton corrections (the sequence of the compiler's own sqrt,
+ * without its rescaling for subnormal and huge arguments, which theta never
+ * reaches); 0 -> 0, negative -> NaN */
+MBM_DEV double sqrtp(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y, h = 0.5 * y;
+  const double r = __builtin_fma(-h, s, 0.5);
+  s = __builtin_fma(s, r, s);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-s, s, x);
+  s = __builtin_fma(d, h, s);
+  d = __builtin_fma(-s, s, x);
+  s = __builtin_fma(d, h, s);
+  return x > 0.0 ? s : (x == 0.0 ? 0.0 : __builtin_nan(""));
+#else
+  return sqrt(x);
+#endif
+}
+
 /* primitive variables {rho, v1, v2, theta, T11, T12, T22, q1, q2} */
 MBM_DEV void primitive(const double *U, double *P)
 {
@@ -151,6 +173,31 @@ MBM_DEV double limit(double d1, double d2)
   }
 }
 
+/* Three limited slopes at once. For the harmonic limiter the three
+ * denominators share one reciprocal (r_k = prod_{j!=k} W_j / prod W_j). */
+template <int LIM>
+MBM_DEV void limit3(const double *d1, const double *d2, double *s)
+{
+  if constexpr (LIM == 4) {
+    double w1[3], w2[3], W[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      w1[k] = fabs(d1[k]) + 1.0e-10;
+      w2[k] = fabs(d2[k]) + 1.0e-10;
+      W[k] = w1[k] + w2[k];
+    }
+    const double p01 = W[0] * W[1];
+    const double inv = rcp(p01 * W[2]);
+    const double i01 = inv * W[2];
+    const double r[3] = { i01 * W[1], i01 * W[0], inv * p01 };
+#pragma unroll
+    for (int k = 0; k < 3; k++) s[k] = (w1[k] * d2[k] + w2[k] * d1[k]) * r[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) s[k] = limit<LIM>(d1[k], d2[k]);
+  }
+}
+
 /* ---------------- Grad-13 flux (Flux.c:79-119) ----------------
  * W: primitives in the face frame {rho, un, ut, theta, Tnn, Tnt, Ttt, qn, qt}.
  * Side holds the products shared by the 9 components of the conservative
@@ -224,8 +271,8 @@ MBM_DEV void hll_comp(const double *WL, const Side &sL, const double *WR, const 
 MBM_DEV void hll(const double *WL, const double *WR, double dtdx, double *Fs,
                  double &smax)
 {
-  const double cL = sqrt((20.0 / 3.0) * WL[3]);
-  const double cR = sqrt((20.0 / 3.0) * WR[3]);
+  const double cL = sqrtp((20.0 / 3.0) * WL[3]);
+  const double cR = sqrtp((20.0 / 3.0) * WR[3]);
   smax = fabs(0.5 * (WL[1] + WR[1])) + (cR > cL ? cR : cL);   /* max(cR,cL), tools.h:3 */
   const double lamL = fmin(WL[1] - cL, 0.0);
   const double lamR = fmax(WR[1] + cR, 0.0);
@@ -292,7 +339,7 @@ MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
 {
   double Q[9];
   primitive(U, Q);
-  const double z = Q[0] * sqrt(Q[3]) * dtkn;
+  const double z = Q[0] * sqrtp(Q[3]) * dtkn;
   double i1, i2;
   damp_inv(z, i1, i2);
   if constexpr (STAGE == 1) {
@@ -312,12 +359,19 @@ MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
     relax(Q, d3, P);
     if constexpr (NEED_INIT) {
       double R[9], Q0[9], V[9];
+      if (force != 0.0) {
+        /* flag 2 (Flux.c:247-253): force, then the identity relaxation */
 #pragma unroll
-      for (int n = 0; n < 9; n++) R[n] = Q[n];
-      R[1] += dt * force;
-      conservative(R, I);
+        for (int n = 0; n < 9; n++) R[n] = Q[n];
+        R[1] += dt * force;
+        conservative(R, I);
+      } else {
+        /* without force flag 2 is the identity: Conservative(Primitive(U)) = U */
+#pragma unroll
+        for (int n = 0; n < 9; n++) I[n] = U[n];
+      }
       primitive(U0, Q0);
-      const double z0 = Q0[0] * sqrt(Q0[3]) * dtkn;
+      const double z0 = Q0[0] * sqrtp(Q0[3]) * dtkn;
       double j1, j2;
       damp_inv(z0, j1, j2);
       relax(Q0, damp0(z0, j1, j2), R);
