@@ -129,6 +129,7 @@ stage_kernel(const StageArgs A)
   const int tix = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6));
   if (tix >= A.ntiles) return;
   const TileDev T = A.tiles[tix];
+  if (T.blk < 0) return;                 /* padding of the XCD ordering */
   const BlockDev B = A.blocks[T.blk];
   const int lx = B.lx, ly = B.ly;
   const int a = T.a0 + lane;
@@ -641,25 +642,53 @@ std::vector<GhostDev> build_ghost_map(const std::vector<mbm_block_desc> &D,
   return m;
 }
 
+/* Tiles of every block in the order block -> row range -> strip, so that
+ * consecutive tiles are horizontally adjacent strips of the same rows. */
 std::vector<TileDev> build_tiles(const std::vector<mbm_block_desc> &D, int rows)
 {
   std::vector<TileDev> t;
   for (int k = 0; k < (int)D.size(); k++) {
     const int lx = D[k].nx + 2 * MBC, ly = D[k].ny + 2 * MBC;
-    /* strip j: lanes [60j, 60j+63], x-faces [60j+1, 60j+61]; need 60j+61 >= lx-3 */
-    for (int a0 = 0;; a0 += STRIP) {
-      for (int r0 = 1; r0 <= ly - 3; r0 += rows) {
+    for (int r0 = 1; r0 <= ly - 3; r0 += rows) {
+      /* strip j: lanes [60j, 60j+63], x-faces [60j+1, 60j+61]; need 60j+61 >= lx-3 */
+      for (int a0 = 0;; a0 += STRIP) {
         TileDev d;
         d.blk = k;
         d.a0 = a0;
         d.r0 = r0;
         d.r1 = std::min(r0 + rows - 1, ly - 3);
         t.push_back(d);
+        if (a0 + 61 >= lx - 3) break;
       }
-      if (a0 + 61 >= lx - 3) break;
     }
   }
   return t;
+}
+
+/* Launch order of the tiles. A workgroup runs WAVES_PER_WG consecutive tiles
+ * (adjacent strips: they share the cache lines at strip edges and the halo
+ * columns), and workgroup w goes to XCD w mod 8. The groups are dealt so
+ * that each XCD receives one contiguous run of groups -- a patch of
+ * neighbouring strips and row ranges -- keeping halo and edge lines in that
+ * XCD's L2. Padding entries (blk = -1) exit at once. */
+std::vector<TileDev> xcd_order(const std::vector<TileDev> &t)
+{
+  constexpr int NXCD = 8;
+  const int G = (int)((t.size() + WAVES_PER_WG - 1) / WAVES_PER_WG);
+  const int per = (G + NXCD - 1) / NXCD;
+  std::vector<TileDev> o((size_t)per * NXCD * WAVES_PER_WG);
+  for (auto &d : o) {
+    d.blk = -1;
+    d.a0 = d.r0 = d.r1 = 0;
+  }
+  for (int g = 0; g < G; g++) {
+    const int x = g / per, j = g % per, wg = j * NXCD + x;
+    for (int w = 0; w < WAVES_PER_WG; w++) {
+      const size_t src = (size_t)g * WAVES_PER_WG + w;
+      if (src < t.size()) o[(size_t)wg * WAVES_PER_WG + w] = t[src];
+    }
+  }
+  return o;
 }
 
 /* Tile height for a deck: every tile is one wavefront and all tiles cost the
@@ -698,7 +727,7 @@ int upload_tiles(mbm_ctx *c)
   const int rows = c->tile_rows > 0 ? c->tile_rows
                                     : auto_tile_rows(c->desc, c->cus * 4 * MBM_STAGE_WAVES);
   c->rows_used = rows;
-  std::vector<TileDev> t = build_tiles(c->desc, rows);
+  std::vector<TileDev> t = xcd_order(build_tiles(c->desc, rows));
   if (c->dtiles) HIPCHK(hipFree(c->dtiles));
   c->dtiles = nullptr;
   HIPCHK(hipMalloc(&c->dtiles, t.size() * sizeof(TileDev)));
