@@ -113,11 +113,17 @@ MBM_DEV void store1b(__amdgpu_buffer_rsrc_t r, int soff, unsigned boff, double v
 
 using namespace mbm;
 
-/* waves per SIMD the stage kernel is compiled for: 1 gives each wave the
- * whole 512-entry register file (256 VGPRs + AGPRs), so nothing spills; the
- * row prefetch and the nine independent components hide the latencies */
+/* 1: carry the upper face state and the next rows' initial values in LDS
+ * (lane-private slots, 54 KiB per workgroup) instead of 36 VGPRs; with that
+ * the kernel fits two waves per SIMD without spills */
+#ifndef MBM_LDS_STATE
+#define MBM_LDS_STATE 1
+#endif
+
+/* waves per SIMD the stage kernel is compiled for: 2 (256 registers per
+ * wave, 220 used) lets one wave's memory waits overlap the other's FP64 work */
 #ifndef MBM_STAGE_WAVES
-#define MBM_STAGE_WAVES 1
+#define MBM_STAGE_WAVES 2
 #endif
 
 template <int STAGE, int LIM>
@@ -171,10 +177,24 @@ stage_kernel(const StageArgs A)
 
   auto row_of = [&](int r) { return r < 0 ? 0 : (r > ly - 1 ? ly - 1 : r); };
 
+#if MBM_LDS_STATE
+  /* lane-private LDS slots for the state carried between rows: the upper
+   * face state of the current row (slot 0) and the initial values of the next
+   * two rows (slots 1, 2 by row parity); keeps them out of registers */
+  __shared__ double sm[WAVES_PER_WG][3][NV][64];
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+#define TUP(n) sm[wv][0][n][lane]
+#define INIT(row, n) sm[wv][1 + ((row) & 1)][n][lane]
+#endif
+
   /* ---- warm-up: rows r0-2 .. r0+1 ---- */
   const int r0 = T.r0;
-  double Pa[NV], Pb[NV], Pc[NV], Pn[NV], In1[NV], acc[NV], Tup[NV];
+  double Pa[NV], Pb[NV], Pc[NV], Pn[NV], acc[NV];
+#if !MBM_LDS_STATE
+  double In1[NV], Tup[NV];
+#endif
   {
+    double Ir[NV];
     int rr = row_of(r0 - 2);
     load9b(rin, pb, cin + rowb * rr, U);
     project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
@@ -189,8 +209,16 @@ stage_kernel(const StageArgs A)
     rr = row_of(r0 + 1);
     load9b(rin, pb, cin + rowb * rr, U);
     if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb * rr, U0);
-    project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, In1, cp);
+    project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp);
     if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+#if MBM_LDS_STATE
+      INIT(r0 + 1, n) = Ir[n];
+#else
+      In1[n] = Ir[n];
+#endif
+    }
 
     double Tprev[NV], Bot[NV], G[NV], s;
     /* slopes in groups of three, exactly as in the march (same rounding) */
@@ -212,7 +240,11 @@ stage_kernel(const StageArgs A)
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         Tprev[g + k] = Pb[g + k] + 0.5 * sp[k];
+#if MBM_LDS_STATE
+        TUP(g + k) = Pc[g + k] + 0.5 * sc[k];
+#else
         Tup[g + k] = Pc[g + k] + 0.5 * sc[k];
+#endif
         Bot[g + k] = Pc[g + k] - 0.5 * sc[k];
       }
     }
@@ -255,15 +287,33 @@ stage_kernel(const StageArgs A)
     __builtin_amdgcn_sched_barrier(0);
 
     /* row r+2: projection */
-    double P2[NV], I2[NV];
+    double P2[NV];
     const int rr = r + 2;
-    project<STAGE, true>(U, U0, col_ring3 || rr == ly - 1, dt, dtkn, force, P2, I2, cp);
+    {
+      double I2[NV];
+      project<STAGE, true>(U, U0, col_ring3 || rr == ly - 1, dt, dtkn, force, P2, I2, cp);
+#if MBM_LDS_STATE
+#pragma unroll
+      for (int n = 0; n < NV; n++) INIT(rr, n) = I2[n];
+#else
+#pragma unroll
+      for (int n = 0; n < NV; n++) U0[n] = I2[n];    /* parked until the row ends */
+#endif
+    }
     if (STAGE == 1 && col_prod && rr <= ly - 2) fold_max(mxp, cp);
     __builtin_amdgcn_sched_barrier(0);
 
     /* y-face r+1/2 */
     {
-      double Tn[NV], Bn[NV], G[NV], s;
+      double Tu[NV], Bn[NV], G[NV], s;
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+#if MBM_LDS_STATE
+        Tu[n] = TUP(n);
+#else
+        Tu[n] = Tup[n];
+#endif
+      }
 #pragma unroll
       for (int g = 0; g < NV; g += 3) {
         double e1[3], e2[3], sy[3];
@@ -275,11 +325,15 @@ stage_kernel(const StageArgs A)
         limit3<LIM>(e1, e2, sy);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          Tn[g + k] = Pn[g + k] + 0.5 * sy[k];
+#if MBM_LDS_STATE
+          TUP(g + k) = Pn[g + k] + 0.5 * sy[k];
+#else
+          Tup[g + k] = Pn[g + k] + 0.5 * sy[k];
+#endif
           Bn[g + k] = Pn[g + k] - 0.5 * sy[k];
         }
       }
-      face<1>(Tup, Bn, dtdy, G, s);
+      face<1>(Tu, Bn, dtdy, G, s);
       if (col_face) fold_max(mxy, s);
       if (st_col && r >= MBC && r <= ly - 1 - MBC) {
         const unsigned ob = cout + rowb * (unsigned)r;
@@ -291,14 +345,21 @@ stage_kernel(const StageArgs A)
       }
 #pragma unroll
       for (int n = 0; n < NV; n++) {
+#if MBM_LDS_STATE
+        acc[n] = INIT(r + 1, n) + G[n];
+#else
         acc[n] = In1[n] + G[n];
-        In1[n] = I2[n];
-        Tup[n] = Tn[n];
+        In1[n] = U0[n];
+#endif
         Pc[n] = Pn[n];
         Pn[n] = P2[n];
       }
     }
   }
+#if MBM_LDS_STATE
+#undef TUP
+#undef INIT
+#endif
 
   /* ---- CFL maxima of this wavefront ---- */
   mxx = wave_max(mxx);
