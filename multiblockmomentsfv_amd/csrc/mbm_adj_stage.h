@@ -87,6 +87,69 @@ MBM_DEV void adj_face_cell(const double *P, const double *gout, double *FxL, dou
   }
 }
 
+/* VJP of the projection of one cell (Main.c:314-336): gP is the adjoint of
+ * its stage primitives, gI (may be null) that of its initial value. Adds to
+ * gU (stage input) and, in stage 2, gU0 (DATA); returns the dJ/dKn term. */
+template <int STAGE>
+MBM_DEV double proj_vjp(const double *U, const double *U0, const double *gP, const double *gI,
+                        double dt, double kn, double force, double *gU, double *gU0)
+{
+  double gkn = 0.0, Q[9];
+  primitive(U, Q);
+  const double dtkn = dt / kn;
+  const double z = Q[0] * sqrtp(Q[3]) * dtkn;
+  double i1, i2;
+  damp_inv(z, i1, i2);
+  double gQ[9] = { 0 }, gz = 0.0, g1 = 0.0, g2 = 0.0;
+  if (STAGE == 1) {
+    relax_vjp(Q, damp1(z, i1, i2), gP, gQ, g1, g2);
+    Damp dz = damp1_dz(z, i1, i2);
+    gz += dz.d1 * g1 + dz.d2 * g2;
+    if (gI) {
+      const Damp d0 = damp0(z, i1, i2);
+      double R[9], gR[9] = { 0 };
+      relax(Q, d0, R);
+      R[1] += dt * force;
+      conservative_vjp(R, gI, gR);
+      g1 = 0.0;
+      g2 = 0.0;
+      relax_vjp(Q, d0, gR, gQ, g1, g2);
+      dz = damp0_dz(z, i1, i2);
+      gz += dz.d1 * g1 + dz.d2 * g2;
+    }
+  } else {
+    Damp d3;
+    d3.d1 = i1;
+    d3.d2 = i2;
+    relax_vjp(Q, d3, gP, gQ, g1, g2);
+    const Damp dz = damp3_dz(i1, i2);
+    gz += dz.d1 * g1 + dz.d2 * g2;
+    if (gI) {
+      double Qf[9];
+      for (int n = 0; n < 9; n++) Qf[n] = Q[n];
+      Qf[1] += dt * force;
+      conservative_vjp(Qf, gI, gQ);
+      /* Data0 part: cons(relax(prim(U0), damp0(z0))) */
+      double Q0[9], R0[9], gR0[9] = { 0 }, gQ0[9] = { 0 };
+      primitive(U0, Q0);
+      const double z0 = Q0[0] * sqrtp(Q0[3]) * dtkn;
+      double j1, j2;
+      damp_inv(z0, j1, j2);
+      const Damp e0 = damp0(z0, j1, j2);
+      relax(Q0, e0, R0);
+      conservative_vjp(R0, gI, gR0);
+      double h1 = 0.0, h2 = 0.0;
+      relax_vjp(Q0, e0, gR0, gQ0, h1, h2);
+      const Damp ez = damp0_dz(z0, j1, j2);
+      z_vjp(Q0, dtkn, kn, ez.d1 * h1 + ez.d2 * h2, gQ0, gkn);
+      primitive_vjp(U0, gQ0, gU0);
+    }
+  }
+  z_vjp(Q, dtkn, kn, gz, gQ, gkn);
+  primitive_vjp(U, gQ, gU);
+  return gkn;
+}
+
 template <int STAGE, int LIM, class BLK>
 MBM_DEV double adj_cell_cell(const double *in, const double *in0, const double *P,
                              const double *FxL, const double *FxR, const double *FyL,
@@ -134,60 +197,13 @@ MBM_DEV double adj_cell_cell(const double *in, const double *in0, const double *
           }
         }
       }
-      double gI[9], U[9], Q[9];
+      double gI[9], U[9], U0[9];
       adj_ld(gout, pl, off, gI);
       const double wI = adj_interior(a, b, lx, ly) ? scale : 0.0;
       for (int n = 0; n < 9; n++) gI[n] *= wI;
       adj_ld(in, pl, off, U);
-      primitive(U, Q);
-      const double dtkn = dt / kn;
-      const double z = Q[0] * sqrtp(Q[3]) * dtkn;
-      double i1, i2;
-      damp_inv(z, i1, i2);
-      double gQ[9] = { 0 }, gz = 0.0, g1 = 0.0, g2 = 0.0;
-      if (STAGE == 1) {
-        relax_vjp(Q, damp1(z, i1, i2), gP, gQ, g1, g2);
-        Damp dz = damp1_dz(z, i1, i2);
-        gz += dz.d1 * g1 + dz.d2 * g2;
-        const Damp d0 = damp0(z, i1, i2);
-        double R[9], gR[9] = { 0 };
-        relax(Q, d0, R);
-        R[1] += dt * force;
-        conservative_vjp(R, gI, gR);
-        g1 = 0.0;
-        g2 = 0.0;
-        relax_vjp(Q, d0, gR, gQ, g1, g2);
-        dz = damp0_dz(z, i1, i2);
-        gz += dz.d1 * g1 + dz.d2 * g2;
-      } else {
-        Damp d3;
-        d3.d1 = i1;
-        d3.d2 = i2;
-        relax_vjp(Q, d3, gP, gQ, g1, g2);
-        const Damp dz = damp3_dz(i1, i2);
-        gz += dz.d1 * g1 + dz.d2 * g2;
-        double Qf[9];
-        for (int n = 0; n < 9; n++) Qf[n] = Q[n];
-        Qf[1] += dt * force;
-        conservative_vjp(Qf, gI, gQ);
-        /* Data0 part: cons(relax(prim(U0), damp0(z0))) */
-        double U0[9], Q0[9], R0[9], gR0[9] = { 0 }, gQ0[9] = { 0 };
-        adj_ld(in0, pl, off, U0);
-        primitive(U0, Q0);
-        const double z0 = Q0[0] * sqrtp(Q0[3]) * dtkn;
-        double j1, j2;
-        damp_inv(z0, j1, j2);
-        const Damp e0 = damp0(z0, j1, j2);
-        relax(Q0, e0, R0);
-        conservative_vjp(R0, gI, gR0);
-        double h1 = 0.0, h2 = 0.0;
-        relax_vjp(Q0, e0, gR0, gQ0, h1, h2);
-        const Damp ez = damp0_dz(z0, j1, j2);
-        z_vjp(Q0, dtkn, kn, ez.d1 * h1 + ez.d2 * h2, gQ0, gkn);
-        primitive_vjp(U0, gQ0, gU0);
-      }
-      z_vjp(Q, dtkn, kn, gz, gQ, gkn);
-      primitive_vjp(U, gQ, gU);
+      if (STAGE == 2) adj_ld(in0, pl, off, U0);
+      gkn += proj_vjp<STAGE>(U, U0, gP, gI, dt, kn, force, gU, gU0);
     }
     if (STAGE == 1) {
       for (int m = 0; m < 9; m++) gin[off + m * pl] += gU[m];

@@ -8,18 +8,27 @@
  *   fill^T gN; gT, gD <- S2^T gN; fill^T gT; gD += S1^T gT.
  * fill^T folds each ghost's adjoint into its source cell (with the wall
  * signs) and clears the ghosts; it is a gather over the inverse ghost map,
- * so the sums are deterministic. S^T runs as three kernels:
- *   adj_prim : recompute the stage primitives P of every cell;
- *   adj_face : per x-face and y-face, the VJP of the reconstruction + HLL
- *              flux with respect to its two face states;
- *   adj_cell : per cell, gather the face-state adjoints, apply the limiter
- *              VJPs of the slopes that use the cell, then the VJP of the
- *              projection and of the initial value (relaxation, primitive
- *              recovery), and dJ/dKn partial sums per workgroup.
- * Checkpointing: DATA is stored every 'stride' steps during the forward
- * run; the reverse sweep recomputes one segment at a time (sqrt(N) memory).
+ * so the sums are deterministic.
+ *
+ * S^T is split by direction, which is exact because the VJP of the
+ * projection is linear in the adjoint of the primitives:
+ *   gU = proj^T(gP_x) + proj^T(gP_y) + init^T(g).
+ *   adj_x : row-local; one wavefront per 58 owned columns (3 halo lanes per
+ *           side), lane shifts for neighbours. Per row: primitives, x-slopes,
+ *           VJP of the x-faces and of the x-limiter, then proj^T(gP_x) and
+ *           init^T(g) written to gin (and gin0 in stage 2);
+ *   adj_y : one wavefront per 64 columns marching up the rows with the
+ *           primitives, face states, slope arguments and partial adjoints of
+ *           three rows in registers; per row: VJP of the y-face and of the
+ *           y-limiter, and proj^T(gP_y) of the row that is complete, added to
+ *           gin. Every cell is finalised by exactly one lane.
+ * dJ/dKn: one partial sum per wavefront and kernel, summed on the host in a
+ * fixed order. Checkpointing: DATA is stored every 'stride' steps during the
+ * forward run; the reverse sweep recomputes one segment at a time (sqrt(N)
+ * memory).
  */
 #include "mbm_internal.h"
+#include "mbm_dev.h"
 #include "mbm_adj.h"
 #include "mbm_adj_stage.h"
 
@@ -35,65 +44,260 @@ struct mbm_adj_state {
   std::vector<double *> ckpt;     /* ckpt[i]: DATA after i*stride steps */
   std::vector<double *> seg;      /* segment states during the reverse sweep */
   double *g[3] = { nullptr, nullptr, nullptr };   /* gN, gD, gT */
-  double *P = nullptr, *FxL = nullptr, *FxR = nullptr, *FyL = nullptr, *FyR = nullptr;
   double *tmp = nullptr;          /* primal TMP recompute */
+  TileDev *xtiles = nullptr, *ytiles = nullptr;
+  int nxt = 0, nyt = 0;
   unsigned long long *slots = nullptr;
   double *knpart = nullptr;
   int nknpart = 0;
   int *inv_off = nullptr, *inv_list = nullptr, *inv_src_blk = nullptr, *inv_src_cell = nullptr;
   int *ghost_blk = nullptr, *ghost_cell = nullptr;
   int nsrc = 0, nghost = 0;
-  int maxcells = 0;
 };
 
 namespace {
 constexpr int ADJ_THREADS = 256;
 }
 
-template <int STAGE>
-__global__ void __launch_bounds__(ADJ_THREADS) adj_prim(const double *in, double *P,
-                                                        const BlockDev *blocks, double dt,
-                                                        double dtkn, double force)
+/* arguments of the direction-split stage adjoint */
+struct AdjArgs {
+  const double *in, *in0, *gout;
+  double *gin, *gin0;
+  const BlockDev *blocks;
+  const TileDev *tiles;
+  int ntiles;
+  double dt, kn, force, scale;
+  double *knpart;                     /* one slot per tile, accumulated over launches */
+};
+
+constexpr int ADJX_STRIP = 58;        /* owned columns of an adj_x wavefront */
+constexpr int ADJX_ROWS = 8;          /* rows per adj_x wavefront */
+constexpr int ADJY_ROWS = 32;         /* rows per adj_y wavefront (3 rows of warm-up) */
+
+MBM_DEV bool cell_interior(int a, int b, int lx, int ly)
 {
-  const BlockDev B = blocks[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B.lx * B.ly) return;
-  adj_prim_cell<STAGE>(in, P, B, i % B.lx, i / B.lx, dt, dtkn, force);
+  return a >= 3 && a <= lx - 4 && b >= 3 && b <= ly - 4;
 }
 
-template <int LIM>
-__global__ void __launch_bounds__(ADJ_THREADS) adj_face(const double *P, const double *gout,
-                                                        double *FxL, double *FxR, double *FyL,
-                                                        double *FyR, const BlockDev *blocks,
-                                                        double dt, double scale)
-{
-  const BlockDev B = blocks[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B.lx * B.ly) return;
-  adj_face_cell<LIM>(P, gout, FxL, FxR, FyL, FyR, B, i % B.lx, i / B.lx, dt, scale);
-}
-
+/* x-direction part of S^T plus the VJP of the initial value, row by row */
 template <int STAGE, int LIM>
-__global__ void __launch_bounds__(ADJ_THREADS) adj_cell(
-    const double *in, const double *in0, const double *P, const double *FxL, const double *FxR,
-    const double *FyL, const double *FyR, const double *gout, double *gin, double *gin0,
-    const BlockDev *blocks, double dt, double kn, double force, double scale, double *knpart)
+__global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
 {
-  const BlockDev B = blocks[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int tix = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+  if (tix >= A.ntiles) return;
+  const TileDev T = A.tiles[tix];
+  if (T.blk < 0) return;
+  const BlockDev B = A.blocks[T.blk];
+  const int lx = B.lx, ly = B.ly;
+  const int a = T.a0 + lane;
+  const int ac = a < 0 ? 0 : (a > lx - 1 ? lx - 1 : a);
+  const bool own = lane >= 3 && lane <= 60 && a >= 0 && a <= lx - 1;
+  const __amdgpu_buffer_rsrc_t rin = block_rsrc(A.in + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rin0 = block_rsrc((STAGE == 2 ? A.in0 : A.in) + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rgi = block_rsrc(A.gin + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rgi0 = block_rsrc((STAGE == 2 ? A.gin0 : A.gin) + B.off, B.plane);
+  const int pb = (int)(8 * B.plane);
+  const unsigned rowb = 8u * (unsigned)B.pitch, col = 8u * (unsigned)ac;
+  const double dtdx = A.dt / B.dx, dtkn = A.dt / A.kn;
   double gkn = 0.0;
-  if (i < B.lx * B.ly)
-    gkn = adj_cell_cell<STAGE, LIM>(in, in0, P, FxL, FxR, FyL, FyR, gout, gin, gin0, B, i % B.lx,
-                                    i / B.lx, dt, kn, force, scale);
-  /* dJ/dKn: one partial sum per workgroup, fixed order */
-  __shared__ double red[ADJ_THREADS];
-  red[threadIdx.x] = gkn;
-  __syncthreads();
-  for (int s = ADJ_THREADS / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
+
+  for (int r = T.r0; r <= T.r1; r++) {
+    const unsigned boff = col + rowb * (unsigned)r;
+    const bool inner = a >= 1 && a <= lx - 2 && r >= 1 && r <= ly - 2;
+    const bool ring3 = a <= 0 || a >= lx - 1 || r == 0 || r == ly - 1;
+    double U[NV], U0[NV], g[NV], P[NV], dummy[NV], cp;
+    load9b(rin, pb, boff, U);
+    if constexpr (STAGE == 2) load9b(rin0, pb, boff, U0);
+    load9b(rg, pb, boff, g);
+    const double w = cell_interior(a, r, lx, ly) ? A.scale : 0.0;
+#pragma unroll
+    for (int n = 0; n < NV; n++) g[n] *= w;
+    project<STAGE, false>(U, U, ring3, A.dt, dtkn, A.force, P, dummy, cp);
+
+    /* x-slopes, face states; the right state of face a+1/2 comes from lane a+1 */
+    double d1[NV], d2[NV], L[NV], Rn[NV], gF[NV];
+#pragma unroll
+    for (int gq = 0; gq < NV; gq += 3) {
+      double sx[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        d1[gq + k] = P[gq + k] - from_left(P[gq + k]);
+        d2[gq + k] = from_right(P[gq + k]) - P[gq + k];
+      }
+      limit3<LIM>(d1 + gq, d2 + gq, sx);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        L[gq + k] = P[gq + k] + 0.5 * sx[k];
+        Rn[gq + k] = from_right(P[gq + k] - 0.5 * sx[k]);
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NV; n++) gF[n] = from_right(g[n]) - g[n];
+    double gL[NV] = { 0 }, gRn[NV] = { 0 };
+    face_vjp<0>(L, Rn, dtdx, gF, gL, gRn);
+
+    /* gather: own left state, right state (from face a-1/2), slope VJPs of
+     * the slopes at a-1, a, a+1 */
+    double gP[NV];
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+      const double gR = from_left(gRn[n]);
+      double gd1 = 0.0, gd2 = 0.0;
+      limit_vjp<LIM>(d1[n], d2[n], 0.5 * gL[n] - 0.5 * gR, gd1, gd2);
+      gP[n] = gL[n] + gR + gd1 - gd2 - from_right(gd1) + from_left(gd2);
+      if (!inner) gP[n] = 0.0;
+    }
+    if (own) {
+      double gU[NV] = { 0 }, gU0[NV] = { 0 };
+      if (inner) gkn += proj_vjp<STAGE>(U, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
+      if constexpr (STAGE == 1) {
+        double o[NV];
+        load9b(rgi, pb, boff, o);
+#pragma unroll
+        for (int n = 0; n < NV; n++) store1b(rgi, n * pb, boff, o[n] + gU[n]);
+      } else {
+        double o[NV];
+        load9b(rgi0, pb, boff, o);
+#pragma unroll
+        for (int n = 0; n < NV; n++) {
+          store1b(rgi, n * pb, boff, gU[n]);
+          store1b(rgi0, n * pb, boff, o[n] + gU0[n]);
+        }
+      }
+    }
   }
-  if (threadIdx.x == 0) knpart[blockIdx.y * gridDim.x + blockIdx.x] += red[0];
+  gkn = wave_sum(gkn);
+  if (lane == 0) A.knpart[tix] += gkn;
+}
+
+/* y-direction part of S^T: march up the rows of a 64-column strip */
+template <int STAGE, int LIM>
+__global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
+{
+  const int lane = threadIdx.x & 63;
+  const int tix = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+  if (tix >= A.ntiles) return;
+  const TileDev T = A.tiles[tix];
+  if (T.blk < 0) return;
+  const BlockDev B = A.blocks[T.blk];
+  const int lx = B.lx, ly = B.ly;
+  const int a = T.a0 + lane;
+  const int ac = a > lx - 1 ? lx - 1 : a;
+  const bool colin = a >= 1 && a <= lx - 2;
+  const bool colr3 = a == 0 || a >= lx - 1;
+  const __amdgpu_buffer_rsrc_t rin = block_rsrc(A.in + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rgi = block_rsrc(A.gin + B.off, B.plane);
+  const int pb = (int)(8 * B.plane);
+  const unsigned rowb = 8u * (unsigned)B.pitch, col = 8u * (unsigned)ac;
+  const double dtdy = A.dt / B.dy, dtkn = A.dt / A.kn;
+  auto rowc = [&](int r) { return r < 0 ? 0 : (r > ly - 1 ? ly - 1 : r); };
+  auto prims = [&](int r, double *P) {
+    const int rr = rowc(r);
+    double U[NV], dummy[NV], cp;
+    load9b(rin, pb, col + rowb * (unsigned)rr, U);
+    project<STAGE, false>(U, U, colr3 || rr == 0 || rr == ly - 1, A.dt, dtkn, A.force, P, dummy,
+                          cp);
+  };
+  auto gweighted = [&](int r, double *g) {
+    const int rr = rowc(r);
+    load9b(rg, pb, col + rowb * (unsigned)rr, g);
+    const double w = cell_interior(a, r, lx, ly) ? A.scale : 0.0;
+#pragma unroll
+    for (int n = 0; n < NV; n++) g[n] *= w;
+  };
+
+  /* state at k = r0-2: P_k, P_k+1, slope arguments and upper face state of
+   * row k, adjoint of the lower face state of row k, weighted gout of row k,
+   * partial gP of rows k-1 and k */
+  const int r0 = T.r0, r1 = T.r1;
+  int k = r0 - 2;
+  double Pk[NV], Pk1[NV], d1k[NV], d2k[NV], Tk[NV], gBk[NV], gk[NV], gPm[NV], gP0[NV];
+  {
+    double Pm[NV];
+    prims(k - 1, Pm);
+    prims(k, Pk);
+    prims(k + 1, Pk1);
+#pragma unroll
+    for (int gq = 0; gq < NV; gq += 3) {
+      double s3[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        d1k[gq + j] = Pk[gq + j] - Pm[gq + j];
+        d2k[gq + j] = Pk1[gq + j] - Pk[gq + j];
+      }
+      limit3<LIM>(d1k + gq, d2k + gq, s3);
+#pragma unroll
+      for (int j = 0; j < 3; j++) Tk[gq + j] = Pk[gq + j] + 0.5 * s3[j];
+    }
+    gweighted(k, gk);
+#pragma unroll
+    for (int n = 0; n < NV; n++) gBk[n] = gPm[n] = gP0[n] = 0.0;
+  }
+  double gkn = 0.0;
+  for (; k <= r1 + 1; k++) {
+    double P2[NV], g1[NV], e1[NV], e2[NV], T1[NV], B1[NV];
+    prims(k + 2, P2);
+    gweighted(k + 1, g1);
+    /* slope of row k+1 and its face states */
+#pragma unroll
+    for (int gq = 0; gq < NV; gq += 3) {
+      double s3[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        e1[gq + j] = Pk1[gq + j] - Pk[gq + j];
+        e2[gq + j] = P2[gq + j] - Pk1[gq + j];
+      }
+      limit3<LIM>(e1 + gq, e2 + gq, s3);
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        T1[gq + j] = Pk1[gq + j] + 0.5 * s3[j];
+        B1[gq + j] = Pk1[gq + j] - 0.5 * s3[j];
+      }
+    }
+    /* face k+1/2 */
+    double gG[NV], gT[NV] = { 0 }, gB1[NV] = { 0 };
+#pragma unroll
+    for (int n = 0; n < NV; n++) gG[n] = g1[n] - gk[n];
+    face_vjp<1>(Tk, B1, dtdy, gG, gT, gB1);
+    /* slope of row k: T_k = P_k + s/2, B_k = P_k - s/2 */
+    double gP1[NV];
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+      double gd1 = 0.0, gd2 = 0.0;
+      limit_vjp<LIM>(d1k[n], d2k[n], 0.5 * gT[n] - 0.5 * gBk[n], gd1, gd2);
+      gP0[n] += gT[n] + gBk[n] + gd1 - gd2;
+      gPm[n] -= gd1;
+      gP1[n] = gd2;
+    }
+    /* row k-1 is complete */
+    const int rf = k - 1;
+    if (rf >= r0 && colin && rf >= 1 && rf <= ly - 2) {
+      const unsigned boff = col + rowb * (unsigned)rf;
+      double U[NV], gU[NV] = { 0 }, o[NV];
+      load9b(rin, pb, boff, U);
+      gkn += proj_vjp<STAGE>(U, U, gPm, nullptr, A.dt, A.kn, A.force, gU, gU);
+      load9b(rgi, pb, boff, o);
+#pragma unroll
+      for (int n = 0; n < NV; n++) store1b(rgi, n * pb, boff, o[n] + gU[n]);
+    }
+#pragma unroll
+    for (int n = 0; n < NV; n++) {
+      Pk[n] = Pk1[n];
+      Pk1[n] = P2[n];
+      d1k[n] = e1[n];
+      d2k[n] = e2[n];
+      Tk[n] = T1[n];
+      gBk[n] = gB1[n];
+      gk[n] = g1[n];
+      gPm[n] = gP0[n];
+      gP0[n] = gP1[n];
+    }
+  }
+  gkn = wave_sum(gkn);
+  if (lane == 0) knpart[tix] += gkn;
 }
 
 /* fill^T: g(src) += sum over its ghosts of sign * g(ghost) */
@@ -148,17 +352,28 @@ int adj_setup(mbm_ctx *c)
   c->adj = A;
   int rc;
   for (int k = 0; k < 3; k++) if ((rc = alloc_layer(c, &A->g[k]))) return rc;
-  if ((rc = alloc_layer(c, &A->P))) return rc;
-  if ((rc = alloc_layer(c, &A->FxL))) return rc;
-  if ((rc = alloc_layer(c, &A->FxR))) return rc;
-  if ((rc = alloc_layer(c, &A->FyL))) return rc;
-  if ((rc = alloc_layer(c, &A->FyR))) return rc;
   if ((rc = alloc_layer(c, &A->tmp))) return rc;
   HIPCHK(hipMalloc(&A->slots, (size_t)6 * c->nb * sizeof(unsigned long long)));
   HIPCHK(hipMemsetAsync(A->slots, 0, (size_t)6 * c->nb * sizeof(unsigned long long), c->stream));
-  for (int k = 0; k < c->nb; k++)
-    A->maxcells = std::max(A->maxcells, c->hblk[k].lx * c->hblk[k].ly);
-  A->nknpart = ((A->maxcells + ADJ_THREADS - 1) / ADJ_THREADS) * c->nb;
+  /* adj_x: 58 owned columns per wavefront (lanes 3..60), every row of the
+   * block; adj_y: 64 columns, rows 1..ly-2 in runs of ADJY_ROWS */
+  std::vector<TileDev> xt, yt;
+  for (int k = 0; k < c->nb; k++) {
+    const BlockDev &B = c->hblk[k];
+    for (int r0 = 0; r0 <= B.ly - 1; r0 += ADJX_ROWS)
+      for (int a0 = -3; a0 + 3 <= B.lx - 1; a0 += ADJX_STRIP)
+        xt.push_back(TileDev{ k, a0, r0, std::min(r0 + ADJX_ROWS - 1, B.ly - 1) });
+    for (int r0 = 1; r0 <= B.ly - 2; r0 += ADJY_ROWS)
+      for (int a0 = 0; a0 <= B.lx - 2; a0 += 64)
+        yt.push_back(TileDev{ k, a0, r0, std::min(r0 + ADJY_ROWS - 1, B.ly - 2) });
+  }
+  A->nxt = (int)xt.size();
+  A->nyt = (int)yt.size();
+  HIPCHK(hipMalloc(&A->xtiles, xt.size() * sizeof(TileDev)));
+  HIPCHK(hipMalloc(&A->ytiles, yt.size() * sizeof(TileDev)));
+  HIPCHK(hipMemcpy(A->xtiles, xt.data(), xt.size() * sizeof(TileDev), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(A->ytiles, yt.data(), yt.size() * sizeof(TileDev), hipMemcpyHostToDevice));
+  A->nknpart = A->nxt + A->nyt;
   HIPCHK(hipMalloc(&A->knpart, A->nknpart * sizeof(double)));
   /* inverse ghost map: CSR over source cells */
   std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
@@ -207,15 +422,26 @@ int stage_adjoint_lim(mbm_ctx *c, const double *in, const double *in0, const dou
                       double *gin, double *gin0, double dt)
 {
   mbm_adj_state *A = c->adj;
-  const dim3 grid((A->maxcells + ADJ_THREADS - 1) / ADJ_THREADS, c->nb), block(ADJ_THREADS);
-  const double scale = STAGE == 2 ? 0.5 : 1.0;
-  hipLaunchKernelGGL(adj_prim<STAGE>, grid, block, 0, c->stream, in, A->P, c->dblk, dt,
-                     dt / c->par.kn, c->par.force);
-  hipLaunchKernelGGL(adj_face<LIM>, grid, block, 0, c->stream, A->P, gout, A->FxL, A->FxR,
-                     A->FyL, A->FyR, c->dblk, dt, scale);
-  hipLaunchKernelGGL((adj_cell<STAGE, LIM>), grid, block, 0, c->stream, in, in0, A->P, A->FxL,
-                     A->FxR, A->FyL, A->FyR, gout, gin, gin0, c->dblk, dt, c->par.kn,
-                     c->par.force, scale, A->knpart);
+  AdjArgs X{};
+  X.in = in;
+  X.in0 = in0;
+  X.gout = gout;
+  X.gin = gin;
+  X.gin0 = gin0;
+  X.blocks = c->dblk;
+  X.dt = dt;
+  X.kn = c->par.kn;
+  X.force = c->par.force;
+  X.scale = STAGE == 2 ? 0.5 : 1.0;
+  X.knpart = A->knpart;
+  X.tiles = A->xtiles;
+  X.ntiles = A->nxt;
+  hipLaunchKernelGGL((adj_x<STAGE, LIM>), dim3(A->nxt), dim3(64), 0, c->stream, X);
+  AdjArgs Y = X;
+  Y.tiles = A->ytiles;
+  Y.ntiles = A->nyt;
+  hipLaunchKernelGGL((adj_y<STAGE, LIM>), dim3(A->nyt), dim3(64), 0, c->stream, Y,
+                     A->knpart + A->nxt);
   HIPCHK(hipGetLastError());
   return MBM_OK;
 }
@@ -403,7 +629,9 @@ extern "C" int mbm_adj_release(mbm_ctx *c)
   mbm_adj_state *A = c->adj;
   (void)hipStreamSynchronize(c->stream);
   free_checkpoints(A);
-  for (double *p : { A->g[0], A->g[1], A->g[2], A->P, A->FxL, A->FxR, A->FyL, A->FyR, A->tmp })
+  for (double *p : { A->g[0], A->g[1], A->g[2], A->tmp })
+    if (p) (void)hipFree(p);
+  for (TileDev *p : { A->xtiles, A->ytiles })
     if (p) (void)hipFree(p);
   if (A->slots) (void)hipFree(A->slots);
   if (A->knpart) (void)hipFree(A->knpart);

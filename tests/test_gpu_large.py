@@ -30,6 +30,23 @@ def test_square4_1024_against_oracle(mbm):
     assert err.max() <= REL_LINF_TOL, err
 
 
+def test_lshape_full_size_against_oracle(mbm):
+    """The L-shaped 3-block deck at full size (2048x1024, 1024x1024, 1024x2048
+    cells, walls, Kn 0.05, minmod): two steps against the oracle."""
+    deck = mbm.read_deck(os.path.join(ROOT, "multiblockmomentsfv_amd", "decks", "lshape_2048.txt"))
+    assert deck.interior_cells() == 5242880
+    eng = mbm.Engine.from_deck(deck)
+    eng.init()
+    orc = oracle.Oracle(deck.blocks, kn=deck.kn, limiter=deck.limiter, threads=min(8, os.cpu_count() or 1))
+    orc.init()
+    t, n, _, log = eng.steps(0.0, deck.tout[0], deck.cfl, maxsteps=2)
+    n2, log2 = orc.steps(deck.tout[0], deck.cfl, maxsteps=2)
+    assert n == n2 == 2
+    assert np.allclose(log[:, :3], log2[:, :3], rtol=REL_LINF_TOL, atol=0)
+    err = rel_linf(eng.download(0), orc.get(0))
+    assert err.max() <= REL_LINF_TOL, err
+
+
 def test_split_block_is_bitwise_identical(mbm):
     one = [dict(nx=2048, ny=1024, xmin=-1.0, xmax=1.0, ymin=-0.5, ymax=0.5, bc=[0, 0, -1, 0])]
     two = [dict(nx=1024, ny=1024, xmin=-1.0, xmax=0.0, ymin=-0.5, ymax=0.5, bc=[0, 2, -1, 0]),
