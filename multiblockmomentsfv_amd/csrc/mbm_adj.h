@@ -339,13 +339,14 @@ MBM_DEV void hll_vjp(const double *WL, const double *WR, double dtdx, const doub
     const double galph = -s * rcp(alph) * gs;
     glamR += galph;
     glamL -= galph;
+    /* dc/dtheta = k / (2c) */
     if (WL[1] - cL < 0.0) {
       gWL[1] += glamL;
-      gWL[3] -= glamL * k * 0.5 / cL;
+      gWL[3] -= glamL * k * 0.5 * rcp(cL);
     }
     if (WR[1] + cR > 0.0) {
       gWR[1] += glamR;
-      gWR[3] += glamR * k * 0.5 / cR;
+      gWR[3] += glamR * k * 0.5 * rcp(cR);
     }
   }
   uf_vjp(WL, guL, gfL, gWL);

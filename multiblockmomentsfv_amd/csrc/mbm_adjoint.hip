@@ -24,8 +24,8 @@
  *           gin. Every cell is finalised by exactly one lane.
  * dJ/dKn: one partial sum per wavefront and kernel, summed on the host in a
  * fixed order. Checkpointing: DATA is stored every 'stride' steps during the
- * forward run; the reverse sweep recomputes one segment at a time (sqrt(N)
- * memory).
+ * forward run; the reverse sweep recomputes one segment at a time and keeps
+ * its DATA and stage states (memory: N/stride + 2*stride layers).
  */
 #include "mbm_internal.h"
 #include "mbm_dev.h"
@@ -44,7 +44,6 @@ struct mbm_adj_state {
   std::vector<double *> ckpt;     /* ckpt[i]: DATA after i*stride steps */
   std::vector<double *> seg;      /* segment states during the reverse sweep */
   double *g[3] = { nullptr, nullptr, nullptr };   /* gN, gD, gT */
-  double *tmp = nullptr;          /* primal TMP recompute */
   TileDev *xtiles = nullptr, *ytiles = nullptr;
   int nxt = 0, nyt = 0;
   unsigned long long *slots = nullptr;
@@ -71,8 +70,6 @@ struct AdjArgs {
 };
 
 constexpr int ADJX_STRIP = 58;        /* owned columns of an adj_x wavefront */
-constexpr int ADJX_ROWS = 8;          /* rows per adj_x wavefront */
-constexpr int ADJY_ROWS = 32;         /* rows per adj_y wavefront (3 rows of warm-up) */
 
 MBM_DEV bool cell_interior(int a, int b, int lx, int ly)
 {
@@ -150,6 +147,13 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
       if (!inner) gP[n] = 0.0;
     }
     if (own) {
+      /* reload the cell's state and adjoint (cache hits) instead of keeping
+       * them live across the face VJP */
+      load9b(rin, pb, boff, U);
+      if constexpr (STAGE == 2) load9b(rin0, pb, boff, U0);
+      load9b(rg, pb, boff, g);
+#pragma unroll
+      for (int n = 0; n < NV; n++) g[n] *= w;
       double gU[NV] = { 0 }, gU0[NV] = { 0 };
       if (inner) gkn += proj_vjp<STAGE>(U, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
       if constexpr (STAGE == 1) {
@@ -352,20 +356,43 @@ int adj_setup(mbm_ctx *c)
   c->adj = A;
   int rc;
   for (int k = 0; k < 3; k++) if ((rc = alloc_layer(c, &A->g[k]))) return rc;
-  if ((rc = alloc_layer(c, &A->tmp))) return rc;
   HIPCHK(hipMalloc(&A->slots, (size_t)6 * c->nb * sizeof(unsigned long long)));
   HIPCHK(hipMemsetAsync(A->slots, 0, (size_t)6 * c->nb * sizeof(unsigned long long), c->stream));
   /* adj_x: 58 owned columns per wavefront (lanes 3..60), every row of the
-   * block; adj_y: 64 columns, rows 1..ly-2 in runs of ADJY_ROWS */
+   * block; adj_y: 64 columns, rows 1..ly-2. Both run one wavefront per SIMD;
+   * the rows per wavefront minimise rounds x (rows + warm-up) over the
+   * device's wave slots (adj_y spends three rows of warm-up per run). */
+  const int slots = c->cus * 4;
+  auto pick_rows = [&](int strip, int first, int lastoff, int warm) {
+    int best = 8;
+    double bestc = 1e300;
+    for (int R = 4; R <= 256; R++) {
+      long long n = 0;
+      int maxrows = 0;
+      for (int k = 0; k < c->nb; k++) {
+        const BlockDev &B = c->hblk[k];
+        const int rows = B.ly - lastoff - first;
+        maxrows = std::max(maxrows, rows);
+        n += (long long)((B.lx + 3 + strip - 1) / strip) * ((rows + R - 1) / R);
+      }
+      const double cost = (double)((n + slots - 1) / slots) * (std::min(R, maxrows) + warm);
+      if (cost < bestc - 1e-9) {
+        bestc = cost;
+        best = R;
+      }
+    }
+    return best;
+  };
+  const int rx = pick_rows(ADJX_STRIP, 0, 0, 0), ry = pick_rows(64, 1, 1, 3);
   std::vector<TileDev> xt, yt;
   for (int k = 0; k < c->nb; k++) {
     const BlockDev &B = c->hblk[k];
-    for (int r0 = 0; r0 <= B.ly - 1; r0 += ADJX_ROWS)
+    for (int r0 = 0; r0 <= B.ly - 1; r0 += rx)
       for (int a0 = -3; a0 + 3 <= B.lx - 1; a0 += ADJX_STRIP)
-        xt.push_back(TileDev{ k, a0, r0, std::min(r0 + ADJX_ROWS - 1, B.ly - 1) });
-    for (int r0 = 1; r0 <= B.ly - 2; r0 += ADJY_ROWS)
+        xt.push_back(TileDev{ k, a0, r0, std::min(r0 + rx - 1, B.ly - 1) });
+    for (int r0 = 1; r0 <= B.ly - 2; r0 += ry)
       for (int a0 = 0; a0 <= B.lx - 2; a0 += 64)
-        yt.push_back(TileDev{ k, a0, r0, std::min(r0 + ADJY_ROWS - 1, B.ly - 2) });
+        yt.push_back(TileDev{ k, a0, r0, std::min(r0 + ry - 1, B.ly - 2) });
   }
   A->nxt = (int)xt.size();
   A->nyt = (int)yt.size();
@@ -459,27 +486,35 @@ int stage_adjoint(mbm_ctx *c, const double *in, const double *in0, const double 
   }
 }
 
-/* forward step between explicit buffers (frozen dt, no controller) */
-int step_buffers(mbm_ctx *c, const double *D, double *N, double dt)
+/* stage 1 of a step with its ghost fill: D -> T */
+int stage1_buffers(mbm_ctx *c, const double *D, double *T, double dt)
 {
   mbm_adj_state *A = c->adj;
   int rc;
-  if ((rc = enqueue_stage(c, 1, D, D, A->tmp, dt, A->slots))) return rc;
-  if ((rc = fill_buffer(c, A->tmp))) return rc;
-  if ((rc = enqueue_stage(c, 2, A->tmp, D, N, dt, A->slots + 3 * c->nb))) return rc;
+  if ((rc = enqueue_stage(c, 1, D, D, T, dt, A->slots))) return rc;
+  return fill_buffer(c, T);
+}
+
+/* forward step between explicit buffers (frozen dt, no controller); the
+ * intermediate stage state is left in T */
+int step_buffers(mbm_ctx *c, const double *D, double *T, double *N, double dt)
+{
+  mbm_adj_state *A = c->adj;
+  int rc;
+  if ((rc = stage1_buffers(c, D, T, dt))) return rc;
+  if ((rc = enqueue_stage(c, 2, T, D, N, dt, A->slots + 3 * c->nb))) return rc;
   return fill_buffer(c, N);
 }
 
-/* reverse of one step: gN (in, destroyed) -> gD (out) */
-int step_adjoint(mbm_ctx *c, const double *D, double dt, double *gN, double *gD)
+/* reverse of one step at DATA = D with stage state T: gN (in, destroyed)
+ * -> gD (out) */
+int step_adjoint(mbm_ctx *c, const double *D, const double *T, double dt, double *gN, double *gD)
 {
   mbm_adj_state *A = c->adj;
   int rc;
-  if ((rc = enqueue_stage(c, 1, D, D, A->tmp, dt, A->slots))) return rc;
-  if ((rc = fill_buffer(c, A->tmp))) return rc;
   if ((rc = fill_adjoint(c, gN))) return rc;
   HIPCHK(hipMemsetAsync(gD, 0, c->layer_elems * sizeof(double), c->stream));
-  if ((rc = stage_adjoint<2>(c, A->tmp, D, gN, A->g[2], gD, dt))) return rc;
+  if ((rc = stage_adjoint<2>(c, T, D, gN, A->g[2], gD, dt))) return rc;
   if ((rc = fill_adjoint(c, A->g[2]))) return rc;
   return stage_adjoint<1>(c, D, D, A->g[2], gD, nullptr, dt);
 }
@@ -594,20 +629,23 @@ extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial
   int rc;
   HIPCHK(hipMemcpyAsync(A->g[0], g_final, bytes, hipMemcpyDefault, c->stream));
   HIPCHK(hipMemsetAsync(A->knpart, 0, A->nknpart * sizeof(double), c->stream));
-  while ((int)A->seg.size() < S) {
+  /* per segment: the DATA state and the stage state of every step */
+  while ((int)A->seg.size() < 2 * S) {
     double *p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) return set_err(MBM_E_NOMEM, "adjoint: segment allocation failed");
     A->seg.push_back(p);
   }
+  double **sD = A->seg.data(), **sT = A->seg.data() + S;
   double *gN = A->g[0], *gD = A->g[1];
   const int nseg = (N + S - 1) / S;
   for (int s = nseg - 1; s >= 0; s--) {
     const int k0 = s * S, k1 = std::min(N, k0 + S);
-    HIPCHK(hipMemcpyAsync(A->seg[0], A->ckpt[s], bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(sD[0], A->ckpt[s], bytes, hipMemcpyDeviceToDevice, c->stream));
     for (int k = k0; k < k1 - 1; k++)
-      if ((rc = step_buffers(c, A->seg[k - k0], A->seg[k - k0 + 1], A->dts[k]))) return rc;
+      if ((rc = step_buffers(c, sD[k - k0], sT[k - k0], sD[k - k0 + 1], A->dts[k]))) return rc;
+    if ((rc = stage1_buffers(c, sD[k1 - 1 - k0], sT[k1 - 1 - k0], A->dts[k1 - 1]))) return rc;
     for (int k = k1 - 1; k >= k0; k--) {
-      if ((rc = step_adjoint(c, A->seg[k - k0], A->dts[k], gN, gD))) return rc;
+      if ((rc = step_adjoint(c, sD[k - k0], sT[k - k0], A->dts[k], gN, gD))) return rc;
       std::swap(gN, gD);
     }
   }
@@ -629,7 +667,7 @@ extern "C" int mbm_adj_release(mbm_ctx *c)
   mbm_adj_state *A = c->adj;
   (void)hipStreamSynchronize(c->stream);
   free_checkpoints(A);
-  for (double *p : { A->g[0], A->g[1], A->g[2], A->tmp })
+  for (double *p : { A->g[0], A->g[1], A->g[2] })
     if (p) (void)hipFree(p);
   for (TileDev *p : { A->xtiles, A->ytiles })
     if (p) (void)hipFree(p);
