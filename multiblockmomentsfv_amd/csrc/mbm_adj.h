@@ -188,6 +188,38 @@ MBM_DEV void limit_vjp(double d1, double d2, double gs, double &gd1, double &gd2
   }
 }
 
+/* three limiter VJPs at once; for the harmonic limiter the three
+ * denominators share one reciprocal (as limit3 in the forward) */
+template <int LIM>
+MBM_DEV void limit_vjp3(const double *d1, const double *d2, const double *gs, double *gd1,
+                        double *gd2)
+{
+  if constexpr (LIM == 4) {
+    double w1[3], w2[3], W[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      w1[k] = fabs(d1[k]) + 1.0e-10;
+      w2[k] = fabs(d2[k]) + 1.0e-10;
+      W[k] = w1[k] + w2[k];
+    }
+    const double p01 = W[0] * W[1];
+    const double inv = rcp(p01 * W[2]);
+    const double i01 = inv * W[2];
+    const double r[3] = { i01 * W[1], i01 * W[0], inv * p01 };
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double iW2g = r[k] * r[k] * gs[k];
+      const double s1 = d1[k] > 0.0 ? 1.0 : (d1[k] < 0.0 ? -1.0 : 0.0);
+      const double s2 = d2[k] > 0.0 ? 1.0 : (d2[k] < 0.0 ? -1.0 : 0.0);
+      gd1[k] += w2[k] * (W[k] + s1 * (d2[k] - d1[k])) * iW2g;
+      gd2[k] += w1[k] * (W[k] + s2 * (d1[k] - d2[k])) * iW2g;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) limit_vjp<LIM>(d1[k], d2[k], gs[k], gd1[k], gd2[k]);
+  }
+}
+
 /* ---------------- (U, F) of one face state ---------------- */
 /* gW += gu^T dU/dW + gf^T dF/dW for the frame-local state W */
 MBM_DEV void uf_vjp(const double *W, const double *gu, const double *gf, double *gW)

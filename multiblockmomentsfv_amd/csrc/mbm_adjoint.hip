@@ -50,7 +50,7 @@ struct mbm_adj_state {
   double *knpart = nullptr;
   int nknpart = 0;
   int *inv_off = nullptr, *inv_list = nullptr, *inv_src_blk = nullptr, *inv_src_cell = nullptr;
-  int *ghost_blk = nullptr, *ghost_cell = nullptr;
+  int *dir_list = nullptr, ndir = 0;   /* Dirichlet ghosts (no source) */
   int nsrc = 0, nghost = 0;
 };
 
@@ -139,12 +139,20 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
      * the slopes at a-1, a, a+1 */
     double gP[NV];
 #pragma unroll
-    for (int n = 0; n < NV; n++) {
-      const double gR = from_left(gRn[n]);
-      double gd1 = 0.0, gd2 = 0.0;
-      limit_vjp<LIM>(d1[n], d2[n], 0.5 * gL[n] - 0.5 * gR, gd1, gd2);
-      gP[n] = gL[n] + gR + gd1 - gd2 - from_right(gd1) + from_left(gd2);
-      if (!inner) gP[n] = 0.0;
+    for (int gq = 0; gq < NV; gq += 3) {
+      double gR[3], gs[3], gd1[3] = { 0, 0, 0 }, gd2[3] = { 0, 0, 0 };
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        gR[j] = from_left(gRn[gq + j]);
+        gs[j] = 0.5 * gL[gq + j] - 0.5 * gR[j];
+      }
+      limit_vjp3<LIM>(d1 + gq, d2 + gq, gs, gd1, gd2);
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const int n = gq + j;
+        gP[n] = gL[n] + gR[j] + gd1[j] - gd2[j] - from_right(gd1[j]) + from_left(gd2[j]);
+        if (!inner) gP[n] = 0.0;
+      }
     }
     if (own) {
       /* reload the cell's state and adjoint (cache hits) instead of keeping
@@ -162,12 +170,10 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
 #pragma unroll
         for (int n = 0; n < NV; n++) store1b(rgi, n * pb, boff, o[n] + gU[n]);
       } else {
-        double o[NV];
-        load9b(rgi0, pb, boff, o);
 #pragma unroll
         for (int n = 0; n < NV; n++) {
           store1b(rgi, n * pb, boff, gU[n]);
-          store1b(rgi0, n * pb, boff, o[n] + gU0[n]);
+          store1b(rgi0, n * pb, boff, gU0[n]);
         }
       }
     }
@@ -218,7 +224,16 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
    * partial gP of rows k-1 and k */
   const int r0 = T.r0, r1 = T.r1;
   int k = r0 - 2;
-  double Pk[NV], Pk1[NV], d1k[NV], d2k[NV], Tk[NV], gBk[NV], gk[NV], gPm[NV], gP0[NV];
+  /* lane-private LDS slots for the state read once per row: the upper face
+   * state, the lower face state's adjoint, the weighted gout and the slope
+   * arguments of row k (each slot is read before it is rewritten) */
+  __shared__ double sm[5][NV][64];
+  double(&Tk)[NV][64] = sm[0];
+  double(&gBk)[NV][64] = sm[1];
+  double(&gk)[NV][64] = sm[2];
+  double(&d1k)[NV][64] = sm[3];
+  double(&d2k)[NV][64] = sm[4];
+  double Pk[NV], Pk1[NV], gPm[NV], gP0[NV];
   {
     double Pm[NV];
     prims(k - 1, Pm);
@@ -229,16 +244,28 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
       double s3[3];
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        d1k[gq + j] = Pk[gq + j] - Pm[gq + j];
-        d2k[gq + j] = Pk1[gq + j] - Pk[gq + j];
+        d1k[gq + j][lane] = Pk[gq + j] - Pm[gq + j];
+        d2k[gq + j][lane] = Pk1[gq + j] - Pk[gq + j];
       }
-      limit3<LIM>(d1k + gq, d2k + gq, s3);
+      {
+        double a1[3], a2[3];
 #pragma unroll
-      for (int j = 0; j < 3; j++) Tk[gq + j] = Pk[gq + j] + 0.5 * s3[j];
+        for (int j = 0; j < 3; j++) { a1[j] = d1k[gq + j][lane]; a2[j] = d2k[gq + j][lane]; }
+        limit3<LIM>(a1, a2, s3);
+      }
+#pragma unroll
+      for (int j = 0; j < 3; j++) Tk[gq + j][lane] = Pk[gq + j] + 0.5 * s3[j];
     }
-    gweighted(k, gk);
+    {
+      double g0[NV];
+      gweighted(k, g0);
 #pragma unroll
-    for (int n = 0; n < NV; n++) gBk[n] = gPm[n] = gP0[n] = 0.0;
+      for (int n = 0; n < NV; n++) {
+        gk[n][lane] = g0[n];
+        gBk[n][lane] = 0.0;
+        gPm[n] = gP0[n] = 0.0;
+      }
+    }
   }
   double gkn = 0.0;
   for (; k <= r1 + 1; k++) {
@@ -264,17 +291,33 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
     /* face k+1/2 */
     double gG[NV], gT[NV] = { 0 }, gB1[NV] = { 0 };
 #pragma unroll
-    for (int n = 0; n < NV; n++) gG[n] = g1[n] - gk[n];
-    face_vjp<1>(Tk, B1, dtdy, gG, gT, gB1);
+    for (int n = 0; n < NV; n++) gG[n] = g1[n] - gk[n][lane];
+    {
+      double Tl[NV];
+#pragma unroll
+      for (int n = 0; n < NV; n++) Tl[n] = Tk[n][lane];
+      face_vjp<1>(Tl, B1, dtdy, gG, gT, gB1);
+    }
     /* slope of row k: T_k = P_k + s/2, B_k = P_k - s/2 */
     double gP1[NV];
 #pragma unroll
-    for (int n = 0; n < NV; n++) {
-      double gd1 = 0.0, gd2 = 0.0;
-      limit_vjp<LIM>(d1k[n], d2k[n], 0.5 * gT[n] - 0.5 * gBk[n], gd1, gd2);
-      gP0[n] += gT[n] + gBk[n] + gd1 - gd2;
-      gPm[n] -= gd1;
-      gP1[n] = gd2;
+    for (int gq = 0; gq < NV; gq += 3) {
+      double gs[3], gd1[3] = { 0, 0, 0 }, gd2[3] = { 0, 0, 0 };
+#pragma unroll
+      for (int j = 0; j < 3; j++) gs[j] = 0.5 * gT[gq + j] - 0.5 * gBk[gq + j][lane];
+      {
+        double a1[3], a2[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) { a1[j] = d1k[gq + j][lane]; a2[j] = d2k[gq + j][lane]; }
+        limit_vjp3<LIM>(a1, a2, gs, gd1, gd2);
+      }
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const int n = gq + j;
+        gP0[n] += gT[n] + gBk[n][lane] + gd1[j] - gd2[j];
+        gPm[n] -= gd1[j];
+        gP1[n] = gd2[j];
+      }
     }
     /* row k-1 is complete */
     const int rf = k - 1;
@@ -291,11 +334,11 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
     for (int n = 0; n < NV; n++) {
       Pk[n] = Pk1[n];
       Pk1[n] = P2[n];
-      d1k[n] = e1[n];
-      d2k[n] = e2[n];
-      Tk[n] = T1[n];
-      gBk[n] = gB1[n];
-      gk[n] = g1[n];
+      d1k[n][lane] = e1[n];
+      d2k[n][lane] = e2[n];
+      Tk[n][lane] = T1[n];
+      gBk[n][lane] = gB1[n];
+      gk[n][lane] = g1[n];
       gPm[n] = gP0[n];
       gP0[n] = gP1[n];
     }
@@ -304,34 +347,35 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
   if (lane == 0) knpart[tix] += gkn;
 }
 
-/* fill^T: g(src) += sum over its ghosts of sign * g(ghost) */
-__global__ void __launch_bounds__(ADJ_THREADS) adj_fill_gather(
+/* fill^T in one pass: source cell s gathers sign * g(ghost) over the ghost
+ * cells that copy it and clears them (every ghost copies exactly one interior
+ * cell, so the reads and writes of different threads never meet); the
+ * threads past nsrc clear the Dirichlet ghosts, which copy nothing. */
+__global__ void __launch_bounds__(ADJ_THREADS) adj_fill(
     double *g, const BlockDev *blocks, const GhostDev *map, const int *off, const int *list,
-    const int *src_blk, const int *src_cell, int nsrc)
+    const int *src_blk, const int *src_cell, int nsrc, const int *dir, int ndir)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsrc) return;
+  if (s >= nsrc) {
+    if (s - nsrc >= ndir) return;
+    const GhostDev gh = map[dir[s - nsrc]];
+    const BlockDev D = blocks[gh.dblk];
+    for (int m = 0; m < NV; m++) g[D.off + gh.dcell + m * D.plane] = 0.0;
+    return;
+  }
   const BlockDev S = blocks[src_blk[s]];
   double acc[NV] = { 0 };
   for (int j = off[s]; j < off[s + 1]; j++) {
     const GhostDev gh = map[list[j]];
     const BlockDev D = blocks[gh.dblk];
     for (int m = 0; m < NV; m++) {
-      const double v = g[D.off + gh.dcell + m * D.plane];
+      double *p = g + D.off + gh.dcell + m * D.plane;
+      const double v = *p;
       acc[m] += ((gh.sign >> m) & 1) ? -v : v;
+      *p = 0.0;
     }
   }
   for (int m = 0; m < NV; m++) g[S.off + src_cell[s] + m * S.plane] += acc[m];
-}
-
-__global__ void __launch_bounds__(ADJ_THREADS) adj_fill_clear(double *g, const BlockDev *blocks,
-                                                              const GhostDev *map, int n)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const GhostDev gh = map[i];
-  const BlockDev D = blocks[gh.dblk];
-  for (int m = 0; m < NV; m++) g[D.off + gh.dcell + m * D.plane] = 0.0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -405,8 +449,15 @@ int adj_setup(mbm_ctx *c)
   /* inverse ghost map: CSR over source cells */
   std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
   std::map<std::pair<int, int>, std::vector<int>> inv;
-  for (int i = 0; i < (int)gm.size(); i++)
+  std::vector<int> dir;
+  for (int i = 0; i < (int)gm.size(); i++) {
     if (gm[i].sblk >= 0) inv[{ gm[i].sblk, gm[i].scell }].push_back(i);
+    else dir.push_back(i);
+  }
+  A->ndir = (int)dir.size();
+  HIPCHK(hipMalloc(&A->dir_list, std::max<size_t>(1, dir.size()) * sizeof(int)));
+  if (!dir.empty())
+    HIPCHK(hipMemcpy(A->dir_list, dir.data(), dir.size() * sizeof(int), hipMemcpyHostToDevice));
   std::vector<int> off(1, 0), list, sb, sc;
   for (auto &kv : inv) {
     sb.push_back(kv.first.first);
@@ -433,13 +484,11 @@ int adj_setup(mbm_ctx *c)
 int fill_adjoint(mbm_ctx *c, double *g)
 {
   mbm_adj_state *A = c->adj;
-  if (A->nsrc > 0)
-    hipLaunchKernelGGL(adj_fill_gather, dim3((A->nsrc + ADJ_THREADS - 1) / ADJ_THREADS),
-                       dim3(ADJ_THREADS), 0, c->stream, g, c->dblk, c->dghost, A->inv_off,
-                       A->inv_list, A->inv_src_blk, A->inv_src_cell, A->nsrc);
-  if (A->nghost > 0)
-    hipLaunchKernelGGL(adj_fill_clear, dim3((A->nghost + ADJ_THREADS - 1) / ADJ_THREADS),
-                       dim3(ADJ_THREADS), 0, c->stream, g, c->dblk, c->dghost, A->nghost);
+  const int n = A->nsrc + A->ndir;
+  if (n > 0)
+    hipLaunchKernelGGL(adj_fill, dim3((n + ADJ_THREADS - 1) / ADJ_THREADS), dim3(ADJ_THREADS), 0,
+                       c->stream, g, c->dblk, c->dghost, A->inv_off, A->inv_list, A->inv_src_blk,
+                       A->inv_src_cell, A->nsrc, A->dir_list, A->ndir);
   HIPCHK(hipGetLastError());
   return MBM_OK;
 }
@@ -513,7 +562,7 @@ int step_adjoint(mbm_ctx *c, const double *D, const double *T, double dt, double
   mbm_adj_state *A = c->adj;
   int rc;
   if ((rc = fill_adjoint(c, gN))) return rc;
-  HIPCHK(hipMemsetAsync(gD, 0, c->layer_elems * sizeof(double), c->stream));
+  /* stage 2 writes every cell of gT and gD; stage 1 then adds to gD */
   if ((rc = stage_adjoint<2>(c, T, D, gN, A->g[2], gD, dt))) return rc;
   if ((rc = fill_adjoint(c, A->g[2]))) return rc;
   return stage_adjoint<1>(c, D, D, A->g[2], gD, nullptr, dt);
@@ -575,9 +624,9 @@ extern "C" int mbm_adj_forward(mbm_ctx *c, double t0, double tend, double cflmax
   double t = t0;
   int k = 0;
   if (dts) {
+    /* replay: no CFL read-back, the steps queue without host round trips */
     for (k = 0; k < ndt; k++) {
-      double cfl;
-      if ((rc = mbm_time_update(c, dts[k], t, &cfl))) return rc;
+      if ((rc = enqueue_time_update(c, dts[k]))) return rc;
       mbm_commit(c);
       t += dts[k];
       A->dts.push_back(dts[k]);
@@ -673,7 +722,7 @@ extern "C" int mbm_adj_release(mbm_ctx *c)
     if (p) (void)hipFree(p);
   if (A->slots) (void)hipFree(A->slots);
   if (A->knpart) (void)hipFree(A->knpart);
-  for (int *p : { A->inv_off, A->inv_list, A->inv_src_blk, A->inv_src_cell })
+  for (int *p : { A->inv_off, A->inv_list, A->inv_src_blk, A->inv_src_cell, A->dir_list })
     if (p) (void)hipFree(p);
   delete A;
   c->adj = nullptr;

@@ -148,6 +148,7 @@ int enqueue_stage(mbm_ctx *c, int stage, const double *in, const double *in0, do
 mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_first, int ndecks,
                     const mbm_params *deck_par, void *stream, int *rc);
 int device_steps(mbm_ctx *c, int maxchunk);
+int enqueue_time_update(mbm_ctx *c, double dt);   /* both stages and fills, CFL to device */
 int upload_tiles(mbm_ctx *c);
 int validate(const mbm_block_desc *blocks, int nb, const mbm_params *p);
 int enqueue_device_stages(mbm_ctx *c);
