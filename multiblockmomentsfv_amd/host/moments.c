@@ -96,45 +96,59 @@ static double now_ms(const struct timeval *t0)
   return 1.0e3 * (t.tv_sec - t0->tv_sec) + 1.0e-3 * (t.tv_usec - t0->tv_usec);
 }
 
-/* Steps (Main.c:189-244) */
+/* Steps (Main.c:189-244). The whole output interval runs on the device
+ * (mbm_steps: controller, rejection and clamp to tend in a replayed graph);
+ * the lines the reference prints every 20 steps are written afterwards from
+ * the attempt records, in the same format. The perf figure is the interval's
+ * mean time per cell-update, as the reference's running average. */
+#define STEP_CHUNK 1000
+#define REC_CAP 8192
+
 static int steps(mbm_ctx *ctx, int gridsize, double *t, double tend, double cflmax,
                  double cflreg, int *nout)
 {
-  int nstep = 0, nrej = 0, show = 20, minutes = 0, seconds = 0, rc;
-  double cfl, dt = DT_INITIAL, perf = 0.0, diff = 0.0;
+  const int show = 20;
+  int nstep = 0, nrej = 0, minutes = 0, seconds = 0, rc = 0;
+  double dt = DT_INITIAL, perf = 0.0, diff = 0.0;
+  mbm_step_record *rec = malloc(REC_CAP * sizeof *rec);
+  if (!rec) return MBM_E_NOMEM;
   struct timeval t0;
   gettimeofday(&t0, NULL);
   while (*t < tend) {
-    nstep++;
-    if (!(nstep % show)) { printf(" step: %3d...", nstep); fflush(stdout); }
-    do {
-      if ((rc = mbm_time_update(ctx, dt, *t, &cfl))) return rc;
-      if (cfl > cflmax) {
-        dt = dt * cflreg / cfl;
-        if (!(nstep % show))
-          printf("\n   step rejected: CFL = %f (next dt = %f)\n   repeat ...", cfl, dt);
-        nrej++;
-      } else {
-        *t += dt;
-        dt = dt * cflreg / cfl;
-        if (*t + dt > tend) dt = tend - (*t);
+    int nrec = 0, ns = 0, nr = 0;
+    const int base = nstep;
+    if ((rc = mbm_steps(ctx, t, tend, cflmax, cflreg, STEP_CHUNK, &dt, rec, REC_CAP, &nrec, &ns,
+                        &nr)))
+      break;
+    const double ms = now_ms(&t0);
+    const int shown = nrec < REC_CAP ? nrec : REC_CAP;
+    for (int i = 0; i < shown; i++) {
+      const mbm_step_record *r = &rec[i];
+      const int s = base + r->step;
+      const int first = i == 0 || rec[i - 1].step != r->step || rec[i - 1].accepted;
+      const double next_dt = i + 1 < shown ? rec[i + 1].dt : dt;
+      if (s % show) continue;
+      if (first) printf(" step: %3d...", s);
+      if (!r->accepted) {
+        printf("\n   step rejected: CFL = %f (next dt = %f)\n   repeat ...", r->cfl, next_dt);
+        continue;
       }
-    } while (cfl > cflmax);
-    if (!(nstep % show)) {
-      if (dt > 0) {
+      if (next_dt > 0) {
         diff = 0.0;
-        perf = now_ms(&t0) / (1.0e-3 * gridsize * (nstep + nrej));
-        minutes = (int)(1.0e-6 * perf * gridsize * (tend - (*t)) / (60 * dt));
-        seconds = (int)(1.0e-6 * perf * gridsize * (tend - (*t)) / dt) % 60;
+        perf = ms / (1.0e-3 * gridsize * (double)(base + ns + nrej + nr));
+        minutes = (int)(1.0e-6 * perf * gridsize * (tend - r->t) / (60 * next_dt));
+        seconds = (int)(1.0e-6 * perf * gridsize * (tend - r->t) / next_dt) % 60;
       }
       printf(" done.  CFL: %lf, time: %lf, next dt: %lf, diff: %lf, perf = %.2f (ca. %dm%ds)\n",
-             cfl, *t, dt, diff, perf, minutes, seconds);
-      fflush(stdout);
+             r->cfl, r->t, next_dt, diff, perf, minutes, seconds);
     }
-    if ((rc = mbm_commit(ctx))) return rc;
+    fflush(stdout);
+    nstep += ns;
+    nrej += nr;
   }
+  free(rec);
   *nout = nstep;
-  return 0;
+  return rc;
 }
 
 int main(int argc, const char **argv)

@@ -182,3 +182,11 @@ def test_moments_cli_against_golden(mbm, name):
         gold = [l for l in open(ref + ".hdr") if l.startswith("ZONE")]
         assert mine == gold
         assert f"Done after {352 if name == 'square4' else 112} steps." in r.stdout
+        if name == "square4":
+            # the step log (every 20th step: CFL, time, next dt to 6 decimals)
+            # against the reference's own log of this run
+            import re
+            pat = re.compile(r"step:\s*(\d+)\.\.\. done\.  CFL: ([0-9.]+), time: ([0-9.]+), next dt: ([0-9.]+)")
+            mine_log = pat.findall(r.stdout)
+            gold_log = pat.findall(open(os.path.join(GOLDEN, "square4.ref.txt")).read())
+            assert len(gold_log) == 16 and mine_log == gold_log, (mine_log, gold_log)
