@@ -10,20 +10,19 @@
  * signs) and clears the ghosts; it is a gather over the inverse ghost map,
  * so the sums are deterministic.
  *
- * S^T is split by direction, which is exact because the VJP of the
- * projection is linear in the adjoint of the primitives:
- *   gU = proj^T(gP_x) + proj^T(gP_y) + init^T(g).
+ * S^T is split by direction:
+ *   adj_y : one wavefront per 64 columns marching up the rows with the
+ *           primitives of two rows and the partial adjoints of two rows in
+ *           registers (face state, slope arguments, gout of the current row in
+ *           LDS); per row: VJP of the y-face and of the y-limiter. The y-part
+ *           of the adjoint of the stage primitives, gP_y, is written out;
  *   adj_x : row-local; one wavefront per 58 owned columns (3 halo lanes per
  *           side), lane shifts for neighbours. Per row: primitives, x-slopes,
- *           VJP of the x-faces and of the x-limiter, then proj^T(gP_x) and
- *           init^T(g) written to gin (and gin0 in stage 2);
- *   adj_y : one wavefront per 64 columns marching up the rows with the
- *           primitives, face states, slope arguments and partial adjoints of
- *           three rows in registers; per row: VJP of the y-face and of the
- *           y-limiter, and proj^T(gP_y) of the row that is complete, added to
- *           gin. Every cell is finalised by exactly one lane.
- * dJ/dKn: one partial sum per wavefront and kernel, summed on the host in a
- * fixed order. Checkpointing: DATA is stored every 'stride' steps during the
+ *           VJP of the x-faces and of the x-limiter, then the VJP of the
+ *           projection of gP_x + gP_y and of the initial value, written to gin
+ *           (and gin0 in stage 2). Every cell is finalised by exactly one lane.
+ * dJ/dKn: one partial sum per adj_x wavefront, summed on the host in a fixed
+ * order. Checkpointing: DATA is stored every 'stride' steps during the
  * forward run; the reverse sweep recomputes one segment at a time and keeps
  * its DATA and stage states (memory: N/stride + 2*stride layers).
  */
@@ -44,6 +43,7 @@ struct mbm_adj_state {
   std::vector<double *> ckpt;     /* ckpt[i]: DATA after i*stride steps */
   std::vector<double *> seg;      /* segment states during the reverse sweep */
   double *g[3] = { nullptr, nullptr, nullptr };   /* gN, gD, gT */
+  double *gpy = nullptr;          /* y-part of gP of the current stage */
   TileDev *xtiles = nullptr, *ytiles = nullptr;
   int nxt = 0, nyt = 0;
   unsigned long long *slots = nullptr;
@@ -62,6 +62,7 @@ constexpr int ADJ_THREADS = 256;
 struct AdjArgs {
   const double *in, *in0, *gout;
   double *gin, *gin0;
+  double *gpy;                        /* y-part of the adjoint of the stage primitives */
   const BlockDev *blocks;
   const TileDev *tiles;
   int ntiles;
@@ -95,6 +96,7 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
   const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, B.plane);
   const __amdgpu_buffer_rsrc_t rgi = block_rsrc(A.gin + B.off, B.plane);
   const __amdgpu_buffer_rsrc_t rgi0 = block_rsrc((STAGE == 2 ? A.gin0 : A.gin) + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rgy = block_rsrc(A.gpy + B.off, B.plane);
   const int pb = (int)(8 * B.plane);
   const unsigned rowb = 8u * (unsigned)B.pitch, col = 8u * (unsigned)ac;
   const double dtdx = A.dt / B.dx, dtkn = A.dt / A.kn;
@@ -151,8 +153,14 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
       for (int j = 0; j < 3; j++) {
         const int n = gq + j;
         gP[n] = gL[n] + gR[j] + gd1[j] - gd2[j] - from_right(gd1[j]) + from_left(gd2[j]);
-        if (!inner) gP[n] = 0.0;
       }
+    }
+    {
+      /* plus the y-part computed by adj_y */
+      double gy[NV];
+      load9b(rgy, pb, boff, gy);
+#pragma unroll
+      for (int n = 0; n < NV; n++) gP[n] = inner ? gP[n] + gy[n] : 0.0;
     }
     if (own) {
       /* reload the cell's state and adjoint (cache hits) instead of keeping
@@ -184,7 +192,7 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
 
 /* y-direction part of S^T: march up the rows of a 64-column strip */
 template <int STAGE, int LIM>
-__global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
+__global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
 {
   const int lane = threadIdx.x & 63;
   const int tix = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
@@ -199,7 +207,7 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
   const bool colr3 = a == 0 || a >= lx - 1;
   const __amdgpu_buffer_rsrc_t rin = block_rsrc(A.in + B.off, B.plane);
   const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, B.plane);
-  const __amdgpu_buffer_rsrc_t rgi = block_rsrc(A.gin + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rgy = block_rsrc(A.gpy + B.off, B.plane);
   const int pb = (int)(8 * B.plane);
   const unsigned rowb = 8u * (unsigned)B.pitch, col = 8u * (unsigned)ac;
   const double dtdy = A.dt / B.dy, dtkn = A.dt / A.kn;
@@ -267,7 +275,6 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
       }
     }
   }
-  double gkn = 0.0;
   for (; k <= r1 + 1; k++) {
     double P2[NV], g1[NV], e1[NV], e2[NV], T1[NV], B1[NV];
     prims(k + 2, P2);
@@ -319,16 +326,12 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
         gP1[n] = gd2[j];
       }
     }
-    /* row k-1 is complete */
+    /* row k-1 is complete: its y-part goes to gpy for adj_x */
     const int rf = k - 1;
     if (rf >= r0 && colin && rf >= 1 && rf <= ly - 2) {
       const unsigned boff = col + rowb * (unsigned)rf;
-      double U[NV], gU[NV] = { 0 }, o[NV];
-      load9b(rin, pb, boff, U);
-      gkn += proj_vjp<STAGE>(U, U, gPm, nullptr, A.dt, A.kn, A.force, gU, gU);
-      load9b(rgi, pb, boff, o);
 #pragma unroll
-      for (int n = 0; n < NV; n++) store1b(rgi, n * pb, boff, o[n] + gU[n]);
+      for (int n = 0; n < NV; n++) store1b(rgy, n * pb, boff, gPm[n]);
     }
 #pragma unroll
     for (int n = 0; n < NV; n++) {
@@ -343,8 +346,6 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A, double *knpart)
       gP0[n] = gP1[n];
     }
   }
-  gkn = wave_sum(gkn);
-  if (lane == 0) knpart[tix] += gkn;
 }
 
 /* fill^T in one pass: source cell s gathers sign * g(ghost) over the ghost
@@ -400,6 +401,7 @@ int adj_setup(mbm_ctx *c)
   c->adj = A;
   int rc;
   for (int k = 0; k < 3; k++) if ((rc = alloc_layer(c, &A->g[k]))) return rc;
+  if ((rc = alloc_layer(c, &A->gpy))) return rc;
   HIPCHK(hipMalloc(&A->slots, (size_t)6 * c->nb * sizeof(unsigned long long)));
   HIPCHK(hipMemsetAsync(A->slots, 0, (size_t)6 * c->nb * sizeof(unsigned long long), c->stream));
   /* adj_x: 58 owned columns per wavefront (lanes 3..60), every row of the
@@ -444,7 +446,7 @@ int adj_setup(mbm_ctx *c)
   HIPCHK(hipMalloc(&A->ytiles, yt.size() * sizeof(TileDev)));
   HIPCHK(hipMemcpy(A->xtiles, xt.data(), xt.size() * sizeof(TileDev), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(A->ytiles, yt.data(), yt.size() * sizeof(TileDev), hipMemcpyHostToDevice));
-  A->nknpart = A->nxt + A->nyt;
+  A->nknpart = A->nxt;
   HIPCHK(hipMalloc(&A->knpart, A->nknpart * sizeof(double)));
   /* inverse ghost map: CSR over source cells */
   std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
@@ -510,14 +512,14 @@ int stage_adjoint_lim(mbm_ctx *c, const double *in, const double *in0, const dou
   X.force = c->par.force;
   X.scale = STAGE == 2 ? 0.5 : 1.0;
   X.knpart = A->knpart;
-  X.tiles = A->xtiles;
-  X.ntiles = A->nxt;
-  hipLaunchKernelGGL((adj_x<STAGE, LIM>), dim3(A->nxt), dim3(64), 0, c->stream, X);
+  X.gpy = A->gpy;
   AdjArgs Y = X;
   Y.tiles = A->ytiles;
   Y.ntiles = A->nyt;
-  hipLaunchKernelGGL((adj_y<STAGE, LIM>), dim3(A->nyt), dim3(64), 0, c->stream, Y,
-                     A->knpart + A->nxt);
+  hipLaunchKernelGGL((adj_y<STAGE, LIM>), dim3(A->nyt), dim3(64), 0, c->stream, Y);
+  X.tiles = A->xtiles;
+  X.ntiles = A->nxt;
+  hipLaunchKernelGGL((adj_x<STAGE, LIM>), dim3(A->nxt), dim3(64), 0, c->stream, X);
   HIPCHK(hipGetLastError());
   return MBM_OK;
 }
@@ -716,7 +718,7 @@ extern "C" int mbm_adj_release(mbm_ctx *c)
   mbm_adj_state *A = c->adj;
   (void)hipStreamSynchronize(c->stream);
   free_checkpoints(A);
-  for (double *p : { A->g[0], A->g[1], A->g[2] })
+  for (double *p : { A->g[0], A->g[1], A->g[2], A->gpy })
     if (p) (void)hipFree(p);
   for (TileDev *p : { A->xtiles, A->ytiles })
     if (p) (void)hipFree(p);
