@@ -1,5 +1,5 @@
 /*
- * ref_harness.c -- driver linked against the reference's own objects
+ * ref_harness.c -- harness linked against the reference's own objects
  * (oracle/Makefile builds them from the reference sources into oracle/_ref).
  * TEST INFRASTRUCTURE ONLY.
  *
