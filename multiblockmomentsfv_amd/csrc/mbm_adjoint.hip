@@ -33,14 +33,28 @@
 
 #include <algorithm>
 #include <map>
+#include <type_traits>
 
 using namespace mbmi;
 using namespace mbm;
 
+/* inverse ghost map for fill^T: element offsets of component 0 in a layer */
+struct FoldGhost {
+  long long off;
+  int plane, sign;      /* sign: bit m set, component m negated */
+};
+struct FoldSrc {
+  long long off;
+  int plane, j0, j1;    /* ghosts j0..j1-1 copy this cell */
+  int pad;
+};
+
 struct mbm_adj_state {
   int stride = 0;
   std::vector<double> dts;        /* accepted dt of every step */
-  std::vector<double *> ckpt;     /* ckpt[i]: DATA after i*stride steps */
+  /* layers are allocated on first need and kept until mbm_adj_release */
+  std::vector<double *> ckpt;     /* ckpt[i], i < nckpt: DATA after i*stride steps */
+  int nckpt = 0;
   std::vector<double *> seg;      /* segment states during the reverse sweep */
   double *g[3] = { nullptr, nullptr, nullptr };   /* gN, gD, gT */
   double *gpy = nullptr;          /* y-part of gP of the current stage */
@@ -49,9 +63,10 @@ struct mbm_adj_state {
   unsigned long long *slots = nullptr;
   double *knpart = nullptr;
   int nknpart = 0;
-  int *inv_off = nullptr, *inv_list = nullptr, *inv_src_blk = nullptr, *inv_src_cell = nullptr;
-  int *dir_list = nullptr, ndir = 0;   /* Dirichlet ghosts (no source) */
-  int nsrc = 0, nghost = 0;
+  FoldSrc *fold_src = nullptr;     /* cells that ghosts copy, with their ghost ranges */
+  FoldGhost *fold_ghost = nullptr; /* ghosts grouped by source cell */
+  FoldGhost *fold_dir = nullptr;   /* Dirichlet ghosts (no source) */
+  int nsrc = 0, ndir = 0;
 };
 
 namespace {
@@ -77,7 +92,10 @@ MBM_DEV bool cell_interior(int a, int b, int lx, int ly)
   return a >= 3 && a <= lx - 4 && b >= 3 && b <= ly - 4;
 }
 
-/* x-direction part of S^T plus the VJP of the initial value, row by row */
+/* x-direction part of S^T plus the VJP of the initial value, row by row.
+ * The state and adjoint of row r+1 are loaded while row r is computed (two
+ * register sets, alternating), so the loop waits on memory only when a
+ * row's compute is shorter than the load latency. */
 template <int STAGE, int LIM>
 __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
 {
@@ -102,17 +120,26 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
   const double dtdx = A.dt / B.dx, dtkn = A.dt / A.kn;
   double gkn = 0.0;
 
-  for (int r = T.r0; r <= T.r1; r++) {
+  auto fetch = [&](int r, double *U, double *U0, double *g) {
     const unsigned boff = col + rowb * (unsigned)r;
-    const bool inner = a >= 1 && a <= lx - 2 && r >= 1 && r <= ly - 2;
-    const bool ring3 = a <= 0 || a >= lx - 1 || r == 0 || r == ly - 1;
-    double U[NV], U0[NV], g[NV], P[NV], dummy[NV], cp;
     load9b(rin, pb, boff, U);
     if constexpr (STAGE == 2) load9b(rin0, pb, boff, U0);
     load9b(rg, pb, boff, g);
+  };
+  /* row r from (U, U0, gr); loads row rn into (Un, U0n, gn) */
+  auto row = [&](int r, const double *U, const double *U0, const double *gr, int rn, double *Un,
+                 double *U0n, double *gn) {
+    const unsigned boff = col + rowb * (unsigned)r;
+    double gy[NV], o[NV];
+    load9b(rgy, pb, boff, gy);
+    if constexpr (STAGE == 1) load9b(rgi, pb, boff, o);
+    fetch(rn, Un, U0n, gn);
+    const bool inner = a >= 1 && a <= lx - 2 && r >= 1 && r <= ly - 2;
+    const bool ring3 = a <= 0 || a >= lx - 1 || r == 0 || r == ly - 1;
     const double w = cell_interior(a, r, lx, ly) ? A.scale : 0.0;
+    double g[NV], P[NV], dummy[NV], cp;
 #pragma unroll
-    for (int n = 0; n < NV; n++) g[n] *= w;
+    for (int n = 0; n < NV; n++) g[n] = gr[n] * w;
     project<STAGE, false>(U, U, ring3, A.dt, dtkn, A.force, P, dummy, cp);
 
     /* x-slopes, face states; the right state of face a+1/2 comes from lane a+1 */
@@ -155,26 +182,13 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
         gP[n] = gL[n] + gR[j] + gd1[j] - gd2[j] - from_right(gd1[j]) + from_left(gd2[j]);
       }
     }
-    {
-      /* plus the y-part computed by adj_y */
-      double gy[NV];
-      load9b(rgy, pb, boff, gy);
+    /* plus the y-part computed by adj_y */
 #pragma unroll
-      for (int n = 0; n < NV; n++) gP[n] = inner ? gP[n] + gy[n] : 0.0;
-    }
+    for (int n = 0; n < NV; n++) gP[n] = inner ? gP[n] + gy[n] : 0.0;
     if (own) {
-      /* reload the cell's state and adjoint (cache hits) instead of keeping
-       * them live across the face VJP */
-      load9b(rin, pb, boff, U);
-      if constexpr (STAGE == 2) load9b(rin0, pb, boff, U0);
-      load9b(rg, pb, boff, g);
-#pragma unroll
-      for (int n = 0; n < NV; n++) g[n] *= w;
       double gU[NV] = { 0 }, gU0[NV] = { 0 };
       if (inner) gkn += proj_vjp<STAGE>(U, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
       if constexpr (STAGE == 1) {
-        double o[NV];
-        load9b(rgi, pb, boff, o);
 #pragma unroll
         for (int n = 0; n < NV; n++) store1b(rgi, n * pb, boff, o[n] + gU[n]);
       } else {
@@ -185,6 +199,14 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
         }
       }
     }
+  };
+
+  double Ua[NV], U0a[NV], ga[NV], Ub[NV], U0b[NV], gb[NV];
+  fetch(T.r0, Ua, U0a, ga);
+  for (int r = T.r0; r <= T.r1; r += 2) {
+    /* the last row's look-ahead reloads that row (in bounds, unused) */
+    row(r, Ua, U0a, ga, min(r + 1, T.r1), Ub, U0b, gb);
+    if (r + 1 <= T.r1) row(r + 1, Ub, U0b, gb, min(r + 2, T.r1), Ua, U0a, ga);
   }
   gkn = wave_sum(gkn);
   if (lane == 0) A.knpart[tix] += gkn;
@@ -227,97 +249,111 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
     for (int n = 0; n < NV; n++) g[n] *= w;
   };
 
-  /* state at k = r0-2: P_k, P_k+1, slope arguments and upper face state of
-   * row k, adjoint of the lower face state of row k, weighted gout of row k,
-   * partial gP of rows k-1 and k */
+  /* lane-private LDS rows: the stage primitives of rows k-1..k+2 (a ring
+   * indexed by row & 3), the upper face state of row k, the adjoint of its
+   * lower face state and its weighted gout. Every slot is read before it is
+   * rewritten, by the same lane. */
+  __shared__ double sm[7][NV][64];
+  auto Prow = [&](int r) -> double(&)[NV][64] { return sm[(r + 4) & 3]; };
+  double(&Tk)[NV][64] = sm[4];
+  double(&gBk)[NV][64] = sm[5];
+  double(&gk)[NV][64] = sm[6];
+  /* partial gP of rows k-1 and k */
+  double gPm[NV], gP0[NV];
+
+  /* state at k = r0-2 */
   const int r0 = T.r0, r1 = T.r1;
   int k = r0 - 2;
-  /* lane-private LDS slots for the state read once per row: the upper face
-   * state, the lower face state's adjoint, the weighted gout and the slope
-   * arguments of row k (each slot is read before it is rewritten) */
-  __shared__ double sm[5][NV][64];
-  double(&Tk)[NV][64] = sm[0];
-  double(&gBk)[NV][64] = sm[1];
-  double(&gk)[NV][64] = sm[2];
-  double(&d1k)[NV][64] = sm[3];
-  double(&d2k)[NV][64] = sm[4];
-  double Pk[NV], Pk1[NV], gPm[NV], gP0[NV];
   {
-    double Pm[NV];
+    double Pm[NV], Pk[NV], Pk1[NV];
     prims(k - 1, Pm);
     prims(k, Pk);
     prims(k + 1, Pk1);
 #pragma unroll
     for (int gq = 0; gq < NV; gq += 3) {
-      double s3[3];
+      double d1[3], d2[3], s3[3];
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        d1k[gq + j][lane] = Pk[gq + j] - Pm[gq + j];
-        d2k[gq + j][lane] = Pk1[gq + j] - Pk[gq + j];
+        d1[j] = Pk[gq + j] - Pm[gq + j];
+        d2[j] = Pk1[gq + j] - Pk[gq + j];
       }
-      {
-        double a1[3], a2[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) { a1[j] = d1k[gq + j][lane]; a2[j] = d2k[gq + j][lane]; }
-        limit3<LIM>(a1, a2, s3);
-      }
+      limit3<LIM>(d1, d2, s3);
 #pragma unroll
       for (int j = 0; j < 3; j++) Tk[gq + j][lane] = Pk[gq + j] + 0.5 * s3[j];
     }
-    {
-      double g0[NV];
-      gweighted(k, g0);
+    double g0[NV];
+    gweighted(k, g0);
 #pragma unroll
-      for (int n = 0; n < NV; n++) {
-        gk[n][lane] = g0[n];
-        gBk[n][lane] = 0.0;
-        gPm[n] = gP0[n] = 0.0;
-      }
+    for (int n = 0; n < NV; n++) {
+      Prow(k - 1)[n][lane] = Pm[n];
+      Prow(k)[n][lane] = Pk[n];
+      Prow(k + 1)[n][lane] = Pk1[n];
+      gk[n][lane] = g0[n];
+      gBk[n][lane] = 0.0;
+      gPm[n] = gP0[n] = 0.0;
     }
   }
-  for (; k <= r1 + 1; k++) {
-    double P2[NV], g1[NV], e1[NV], e2[NV], T1[NV], B1[NV];
-    prims(k + 2, P2);
-    gweighted(k + 1, g1);
+  /* inputs of iteration kk (the state of row kk+2, gout of row kk+1) are
+   * loaded one iteration ahead, into two alternating register sets */
+  auto fetch = [&](int kk, double *U, double *g) {
+    load9b(rin, pb, col + rowb * (unsigned)rowc(kk + 2), U);
+    load9b(rg, pb, col + rowb * (unsigned)rowc(kk + 1), g);
+  };
+  auto step = [&](int k, const double *Uc, const double *gc, double *Un, double *gn) {
+    double(&Pk)[NV][64] = Prow(k);
+    double(&Pk1)[NV][64] = Prow(k + 1);
+    double(&P2)[NV][64] = Prow(k + 2);   /* slot of row k-2, no longer read */
+    /* face k+1/2: the upper state of row k (replaced below by that of row
+     * k+1) and the gout difference */
+    double Tl[NV], gG[NV];
+    {
+      const int rr = rowc(k + 2);
+      double P[NV], dummy[NV], cp;
+      project<STAGE, false>(Uc, Uc, colr3 || rr == 0 || rr == ly - 1, A.dt, dtkn, A.force, P,
+                            dummy, cp);
+      const double w = cell_interior(a, k + 1, lx, ly) ? A.scale : 0.0;
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        P2[n][lane] = P[n];
+        Tl[n] = Tk[n][lane];
+        const double g1 = gc[n] * w;
+        gG[n] = g1 - gk[n][lane];
+        gk[n][lane] = g1;
+      }
+    }
     /* slope of row k+1 and its face states */
+    double B1[NV];
 #pragma unroll
     for (int gq = 0; gq < NV; gq += 3) {
-      double s3[3];
+      double e1[3], e2[3], s3[3];
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        e1[gq + j] = Pk1[gq + j] - Pk[gq + j];
-        e2[gq + j] = P2[gq + j] - Pk1[gq + j];
+        e1[j] = Pk1[gq + j][lane] - Pk[gq + j][lane];
+        e2[j] = P2[gq + j][lane] - Pk1[gq + j][lane];
       }
-      limit3<LIM>(e1 + gq, e2 + gq, s3);
+      limit3<LIM>(e1, e2, s3);
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        T1[gq + j] = Pk1[gq + j] + 0.5 * s3[j];
-        B1[gq + j] = Pk1[gq + j] - 0.5 * s3[j];
+        Tk[gq + j][lane] = Pk1[gq + j][lane] + 0.5 * s3[j];
+        B1[gq + j] = Pk1[gq + j][lane] - 0.5 * s3[j];
       }
     }
-    /* face k+1/2 */
-    double gG[NV], gT[NV] = { 0 }, gB1[NV] = { 0 };
-#pragma unroll
-    for (int n = 0; n < NV; n++) gG[n] = g1[n] - gk[n][lane];
-    {
-      double Tl[NV];
-#pragma unroll
-      for (int n = 0; n < NV; n++) Tl[n] = Tk[n][lane];
-      face_vjp<1>(Tl, B1, dtdy, gG, gT, gB1);
-    }
+    double gT[NV] = { 0 }, gB1[NV] = { 0 };
+    face_vjp<1>(Tl, B1, dtdy, gG, gT, gB1);
+    fetch(k + 1, Un, gn);
     /* slope of row k: T_k = P_k + s/2, B_k = P_k - s/2 */
+    double(&Pm)[NV][64] = Prow(k - 1);
     double gP1[NV];
 #pragma unroll
     for (int gq = 0; gq < NV; gq += 3) {
-      double gs[3], gd1[3] = { 0, 0, 0 }, gd2[3] = { 0, 0, 0 };
+      double d1[3], d2[3], gs[3], gd1[3] = { 0, 0, 0 }, gd2[3] = { 0, 0, 0 };
 #pragma unroll
-      for (int j = 0; j < 3; j++) gs[j] = 0.5 * gT[gq + j] - 0.5 * gBk[gq + j][lane];
-      {
-        double a1[3], a2[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) { a1[j] = d1k[gq + j][lane]; a2[j] = d2k[gq + j][lane]; }
-        limit_vjp3<LIM>(a1, a2, gs, gd1, gd2);
+      for (int j = 0; j < 3; j++) {
+        d1[j] = Pk[gq + j][lane] - Pm[gq + j][lane];
+        d2[j] = Pk1[gq + j][lane] - Pk[gq + j][lane];
+        gs[j] = 0.5 * gT[gq + j] - 0.5 * gBk[gq + j][lane];
       }
+      limit_vjp3<LIM>(d1, d2, gs, gd1, gd2);
 #pragma unroll
       for (int j = 0; j < 3; j++) {
         const int n = gq + j;
@@ -335,48 +371,47 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
     }
 #pragma unroll
     for (int n = 0; n < NV; n++) {
-      Pk[n] = Pk1[n];
-      Pk1[n] = P2[n];
-      d1k[n][lane] = e1[n];
-      d2k[n][lane] = e2[n];
-      Tk[n][lane] = T1[n];
       gBk[n][lane] = gB1[n];
-      gk[n][lane] = g1[n];
       gPm[n] = gP0[n];
       gP0[n] = gP1[n];
     }
+  };
+  double Ua[NV], ga[NV], Ub[NV], gb[NV];
+  fetch(k, Ua, ga);
+  for (; k <= r1 + 1; k += 2) {
+    step(k, Ua, ga, Ub, gb);
+    if (k + 1 <= r1 + 1) step(k + 1, Ub, gb, Ua, ga);
   }
 }
 
-/* fill^T in one pass: source cell s gathers sign * g(ghost) over the ghost
- * cells that copy it and clears them (every ghost copies exactly one interior
- * cell, so the reads and writes of different threads never meet); the
- * threads past nsrc clear the Dirichlet ghosts, which copy nothing. */
-__global__ void __launch_bounds__(ADJ_THREADS) adj_fill(
-    double *g, const BlockDev *blocks, const GhostDev *map, const int *off, const int *list,
-    const int *src_blk, const int *src_cell, int nsrc, const int *dir, int ndir)
+
+/* fill^T in one pass, one thread per (source cell, component m =
+ * blockIdx.y): the source gathers sign * g(ghost) over the ghost cells that
+ * copy it, in a fixed order, and clears them (every ghost copies exactly one
+ * interior cell, so the reads and writes of different threads never meet);
+ * the threads past nsrc clear the Dirichlet ghosts, which copy nothing. */
+__global__ void __launch_bounds__(ADJ_THREADS) adj_fill(double *g, const FoldSrc *src, int nsrc,
+                                                        const FoldGhost *ghost,
+                                                        const FoldGhost *dir, int ndir)
 {
+  const int m = blockIdx.y;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsrc) {
     if (s - nsrc >= ndir) return;
-    const GhostDev gh = map[dir[s - nsrc]];
-    const BlockDev D = blocks[gh.dblk];
-    for (int m = 0; m < NV; m++) g[D.off + gh.dcell + m * D.plane] = 0.0;
+    const FoldGhost d = dir[s - nsrc];
+    g[d.off + (long long)m * d.plane] = 0.0;
     return;
   }
-  const BlockDev S = blocks[src_blk[s]];
-  double acc[NV] = { 0 };
-  for (int j = off[s]; j < off[s + 1]; j++) {
-    const GhostDev gh = map[list[j]];
-    const BlockDev D = blocks[gh.dblk];
-    for (int m = 0; m < NV; m++) {
-      double *p = g + D.off + gh.dcell + m * D.plane;
-      const double v = *p;
-      acc[m] += ((gh.sign >> m) & 1) ? -v : v;
-      *p = 0.0;
-    }
+  const FoldSrc S = src[s];
+  double acc = 0.0;
+  for (int j = S.j0; j < S.j1; j++) {
+    const FoldGhost q = ghost[j];
+    double *p = g + q.off + (long long)m * q.plane;
+    const double v = *p;
+    acc += ((q.sign >> m) & 1) ? -v : v;
+    *p = 0.0;
   }
-  for (int m = 0; m < NV; m++) g[S.off + src_cell[s] + m * S.plane] += acc[m];
+  g[S.off + (long long)m * S.plane] += acc;
 }
 
 /* ------------------------------------------------------------------ */
@@ -456,30 +491,33 @@ int adj_setup(mbm_ctx *c)
     if (gm[i].sblk >= 0) inv[{ gm[i].sblk, gm[i].scell }].push_back(i);
     else dir.push_back(i);
   }
-  A->ndir = (int)dir.size();
-  HIPCHK(hipMalloc(&A->dir_list, std::max<size_t>(1, dir.size()) * sizeof(int)));
-  if (!dir.empty())
-    HIPCHK(hipMemcpy(A->dir_list, dir.data(), dir.size() * sizeof(int), hipMemcpyHostToDevice));
-  std::vector<int> off(1, 0), list, sb, sc;
+  auto ghost_at = [&](int i) {
+    const GhostDev &q = gm[i];
+    const BlockDev &D = c->hblk[q.dblk];
+    return FoldGhost{ D.off + q.dcell, (int)D.plane, q.sign };
+  };
+  std::vector<FoldGhost> fd, fg;
+  std::vector<FoldSrc> fs;
+  for (int i : dir) fd.push_back(ghost_at(i));
   for (auto &kv : inv) {
-    sb.push_back(kv.first.first);
-    sc.push_back(kv.first.second);
-    for (int g : kv.second) list.push_back(g);
-    off.push_back((int)list.size());
+    const BlockDev &S = c->hblk[kv.first.first];
+    FoldSrc src{ S.off + kv.first.second, (int)S.plane, (int)fg.size(), 0, 0 };
+    for (int i : kv.second) fg.push_back(ghost_at(i));
+    src.j1 = (int)fg.size();
+    fs.push_back(src);
   }
-  A->nsrc = (int)sb.size();
-  A->nghost = (int)gm.size();
-  HIPCHK(hipMalloc(&A->inv_off, off.size() * sizeof(int)));
-  HIPCHK(hipMalloc(&A->inv_list, std::max<size_t>(1, list.size()) * sizeof(int)));
-  HIPCHK(hipMalloc(&A->inv_src_blk, std::max<size_t>(1, sb.size()) * sizeof(int)));
-  HIPCHK(hipMalloc(&A->inv_src_cell, std::max<size_t>(1, sc.size()) * sizeof(int)));
-  HIPCHK(hipMemcpy(A->inv_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
-  if (!list.empty())
-    HIPCHK(hipMemcpy(A->inv_list, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
-  if (!sb.empty()) {
-    HIPCHK(hipMemcpy(A->inv_src_blk, sb.data(), sb.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(A->inv_src_cell, sc.data(), sc.size() * sizeof(int), hipMemcpyHostToDevice));
-  }
+  A->ndir = (int)fd.size();
+  A->nsrc = (int)fs.size();
+  auto upload = [&](auto **dst, const auto &v) -> int {
+    using T = typename std::decay_t<decltype(v)>::value_type;
+    HIPCHK(hipMalloc(dst, std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty())
+      HIPCHK(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MBM_OK;
+  };
+  if ((rc = upload(&A->fold_src, fs)) || (rc = upload(&A->fold_ghost, fg)) ||
+      (rc = upload(&A->fold_dir, fd)))
+    return rc;
   return MBM_OK;
 }
 
@@ -488,9 +526,8 @@ int fill_adjoint(mbm_ctx *c, double *g)
   mbm_adj_state *A = c->adj;
   const int n = A->nsrc + A->ndir;
   if (n > 0)
-    hipLaunchKernelGGL(adj_fill, dim3((n + ADJ_THREADS - 1) / ADJ_THREADS), dim3(ADJ_THREADS), 0,
-                       c->stream, g, c->dblk, c->dghost, A->inv_off, A->inv_list, A->inv_src_blk,
-                       A->inv_src_cell, A->nsrc, A->dir_list, A->ndir);
+    hipLaunchKernelGGL(adj_fill, dim3((n + ADJ_THREADS - 1) / ADJ_THREADS, NV), dim3(ADJ_THREADS),
+                       0, c->stream, g, A->fold_src, A->nsrc, A->fold_ghost, A->fold_dir, A->ndir);
   HIPCHK(hipGetLastError());
   return MBM_OK;
 }
@@ -570,12 +607,26 @@ int step_adjoint(mbm_ctx *c, const double *D, const double *T, double dt, double
   return stage_adjoint<1>(c, D, D, A->g[2], gD, nullptr, dt);
 }
 
+/* layer i of a pool, allocated on first use */
+int pool_layer(mbm_ctx *c, std::vector<double *> &pool, int i, double **p)
+{
+  while ((int)pool.size() <= i) {
+    double *q = nullptr;
+    if (hipMalloc(&q, c->layer_elems * sizeof(double)) != hipSuccess)
+      return set_err(MBM_E_NOMEM, "adjoint: allocation of checkpoint layer %d failed", i);
+    pool.push_back(q);
+  }
+  *p = pool[i];
+  return MBM_OK;
+}
+
 int push_checkpoint(mbm_ctx *c, const double *src)
 {
-  double *p = nullptr;
-  if (hipMalloc(&p, c->layer_elems * sizeof(double)) != hipSuccess)
-    return set_err(MBM_E_NOMEM, "adjoint: checkpoint allocation failed");
-  c->adj->ckpt.push_back(p);
+  mbm_adj_state *A = c->adj;
+  double *p;
+  int rc;
+  if ((rc = pool_layer(c, A->ckpt, A->nckpt, &p))) return rc;
+  A->nckpt++;
   HIPCHK(hipMemcpyAsync(p, src, c->layer_elems * sizeof(double), hipMemcpyDeviceToDevice,
                         c->stream));
   return MBM_OK;
@@ -585,6 +636,7 @@ void free_checkpoints(mbm_adj_state *A)
 {
   for (double *p : A->ckpt) (void)hipFree(p);
   A->ckpt.clear();
+  A->nckpt = 0;
   for (double *p : A->seg) (void)hipFree(p);
   A->seg.clear();
 }
@@ -619,16 +671,18 @@ extern "C" int mbm_adj_forward(mbm_ctx *c, double t0, double tend, double cflmax
   int rc = adj_setup(c);
   if (rc) return rc;
   mbm_adj_state *A = c->adj;
-  free_checkpoints(A);
+  A->nckpt = 0;
   A->dts.clear();
   A->stride = stride;
   if ((rc = push_checkpoint(c, c->layer[c->role[MBM_L_DATA]]))) return rc;
   double t = t0;
   int k = 0;
   if (dts) {
-    /* replay: no CFL read-back, the steps queue without host round trips */
+    /* replay: no CFL, the steps queue without host round trips */
     for (k = 0; k < ndt; k++) {
-      if ((rc = enqueue_time_update(c, dts[k]))) return rc;
+      if ((rc = step_buffers(c, c->layer[c->role[MBM_L_DATA]], c->layer[c->role[MBM_L_TMP]],
+                             c->layer[c->role[MBM_L_NEW]], dts[k])))
+        return rc;
       mbm_commit(c);
       t += dts[k];
       A->dts.push_back(dts[k]);
@@ -673,25 +727,25 @@ extern "C" int mbm_adj_schedule(mbm_ctx *c, double *dts, int cap, int *n)
 
 extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial, double *g_kn)
 {
-  if (!c || !c->adj || c->adj->ckpt.empty()) return set_err(MBM_E_STATE, "no recorded forward run");
+  if (!c || !c->adj || c->adj->nckpt == 0) return set_err(MBM_E_STATE, "no recorded forward run");
   mbm_adj_state *A = c->adj;
   const size_t bytes = c->layer_elems * sizeof(double);
   const int N = (int)A->dts.size(), S = A->stride;
   int rc;
   HIPCHK(hipMemcpyAsync(A->g[0], g_final, bytes, hipMemcpyDefault, c->stream));
   HIPCHK(hipMemsetAsync(A->knpart, 0, A->nknpart * sizeof(double), c->stream));
-  /* per segment: the DATA state and the stage state of every step */
-  while ((int)A->seg.size() < 2 * S) {
-    double *p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) return set_err(MBM_E_NOMEM, "adjoint: segment allocation failed");
-    A->seg.push_back(p);
+  /* per segment: the DATA state and the stage state of every step; the
+   * segment's first DATA state is its checkpoint */
+  std::vector<double *> sD(S), sT(S);
+  for (int i = 0; i < S; i++) {
+    if (i > 0 && (rc = pool_layer(c, A->seg, i - 1, &sD[i]))) return rc;
+    if ((rc = pool_layer(c, A->seg, S - 1 + i, &sT[i]))) return rc;
   }
-  double **sD = A->seg.data(), **sT = A->seg.data() + S;
   double *gN = A->g[0], *gD = A->g[1];
   const int nseg = (N + S - 1) / S;
   for (int s = nseg - 1; s >= 0; s--) {
     const int k0 = s * S, k1 = std::min(N, k0 + S);
-    HIPCHK(hipMemcpyAsync(sD[0], A->ckpt[s], bytes, hipMemcpyDeviceToDevice, c->stream));
+    sD[0] = A->ckpt[s];
     for (int k = k0; k < k1 - 1; k++)
       if ((rc = step_buffers(c, sD[k - k0], sT[k - k0], sD[k - k0 + 1], A->dts[k]))) return rc;
     if ((rc = stage1_buffers(c, sD[k1 - 1 - k0], sT[k1 - 1 - k0], A->dts[k1 - 1]))) return rc;
@@ -724,7 +778,7 @@ extern "C" int mbm_adj_release(mbm_ctx *c)
     if (p) (void)hipFree(p);
   if (A->slots) (void)hipFree(A->slots);
   if (A->knpart) (void)hipFree(A->knpart);
-  for (int *p : { A->inv_off, A->inv_list, A->inv_src_blk, A->inv_src_cell, A->dir_list })
+  for (void *p : { (void *)A->fold_src, (void *)A->fold_ghost, (void *)A->fold_dir })
     if (p) (void)hipFree(p);
   delete A;
   c->adj = nullptr;
