@@ -89,8 +89,11 @@ MBM_DEV void adj_face_cell(const double *P, const double *gout, double *FxL, dou
 
 /* VJP of the projection of one cell (Main.c:314-336): gP is the adjoint of
  * its stage primitives, gI (may be null) that of its initial value. Adds to
- * gU (stage input) and, in stage 2, gU0 (DATA); returns the dJ/dKn term. */
-template <int STAGE>
+ * gU (stage input) and, in stage 2 with DATA0, gU0 (DATA); returns the dJ/dKn
+ * term. Stage 2's initial value is cons(relax(prim(U), damp3)) + the DATA
+ * part cons(relax(prim(U0), damp0)); DATA0 = false leaves out the VJP of the
+ * DATA part (its adjoint is gI itself; see adj_x). */
+template <int STAGE, bool DATA0 = true>
 MBM_DEV double proj_vjp(const double *U, const double *U0, const double *gP, const double *gI,
                         double dt, double kn, double force, double *gU, double *gU0)
 {
@@ -129,6 +132,8 @@ MBM_DEV double proj_vjp(const double *U, const double *U0, const double *gP, con
       for (int n = 0; n < 9; n++) Qf[n] = Q[n];
       Qf[1] += dt * force;
       conservative_vjp(Qf, gI, gQ);
+    }
+    if (gI && DATA0) {
       /* Data0 part: cons(relax(prim(U0), damp0(z0))) */
       double Q0[9], R0[9], gR0[9] = { 0 }, gQ0[9] = { 0 };
       primitive(U0, Q0);

@@ -96,7 +96,7 @@ MBM_DEV bool cell_interior(int a, int b, int lx, int ly)
  * The state and adjoint of row r+1 are loaded while row r is computed (two
  * register sets, alternating), so the loop waits on memory only when a
  * row's compute is shorter than the load latency. */
-template <int STAGE, int LIM>
+template <int STAGE, int LIM, bool MERGE>
 __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
 {
   const int lane = threadIdx.x & 63;
@@ -185,17 +185,25 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
     /* plus the y-part computed by adj_y */
 #pragma unroll
     for (int n = 0; n < NV; n++) gP[n] = inner ? gP[n] + gy[n] : 0.0;
+    /* MERGE (no body force): stage 1's initial value and the DATA part of
+     * stage 2's are the same function of DATA, so their VJPs merge: stage 2
+     * leaves its cotangent g in gin0 (= gD) and stage 1 takes g + gD. */
     if (own) {
       double gU[NV] = { 0 }, gU0[NV] = { 0 };
-      if (inner) gkn += proj_vjp<STAGE>(U, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
       if constexpr (STAGE == 1) {
+        if constexpr (MERGE) {
 #pragma unroll
-        for (int n = 0; n < NV; n++) store1b(rgi, n * pb, boff, o[n] + gU[n]);
+          for (int n = 0; n < NV; n++) g[n] += o[n];
+        }
+        if (inner) gkn += proj_vjp<1>(U, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
+#pragma unroll
+        for (int n = 0; n < NV; n++) store1b(rgi, n * pb, boff, MERGE ? gU[n] : o[n] + gU[n]);
       } else {
+        if (inner) gkn += proj_vjp<2, !MERGE>(U, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
 #pragma unroll
         for (int n = 0; n < NV; n++) {
           store1b(rgi, n * pb, boff, gU[n]);
-          store1b(rgi0, n * pb, boff, gU0[n]);
+          store1b(rgi0, n * pb, boff, MERGE ? g[n] : gU0[n]);
         }
       }
     }
@@ -556,7 +564,10 @@ int stage_adjoint_lim(mbm_ctx *c, const double *in, const double *in0, const dou
   hipLaunchKernelGGL((adj_y<STAGE, LIM>), dim3(A->nyt), dim3(64), 0, c->stream, Y);
   X.tiles = A->xtiles;
   X.ntiles = A->nxt;
-  hipLaunchKernelGGL((adj_x<STAGE, LIM>), dim3(A->nxt), dim3(64), 0, c->stream, X);
+  if (X.force == 0.0)
+    hipLaunchKernelGGL((adj_x<STAGE, LIM, true>), dim3(A->nxt), dim3(64), 0, c->stream, X);
+  else
+    hipLaunchKernelGGL((adj_x<STAGE, LIM, false>), dim3(A->nxt), dim3(64), 0, c->stream, X);
   HIPCHK(hipGetLastError());
   return MBM_OK;
 }
@@ -601,7 +612,9 @@ int step_adjoint(mbm_ctx *c, const double *D, const double *T, double dt, double
   mbm_adj_state *A = c->adj;
   int rc;
   if ((rc = fill_adjoint(c, gN))) return rc;
-  /* stage 2 writes every cell of gT and gD; stage 1 then adds to gD */
+  /* stage 2 writes every cell of gT and gD; stage 1 then adds to gD (or,
+   * without a body force, takes gD as part of its initial value's cotangent
+   * and overwrites it) */
   if ((rc = stage_adjoint<2>(c, T, D, gN, A->g[2], gD, dt))) return rc;
   if ((rc = fill_adjoint(c, A->g[2]))) return rc;
   return stage_adjoint<1>(c, D, D, A->g[2], gD, nullptr, dt);

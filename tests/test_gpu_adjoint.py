@@ -84,10 +84,23 @@ def test_one_step_adjoint_matches_cpu_reference(mbm, name, limiter):
     """GPU adjoint of one step == the CPU evaluation of the same adjoint code
     (tests/adjoint_ref.py, itself checked against central differences of the
     oracle), cell by cell."""
+    one_step_vs_cpu_reference(mbm, name, limiter, 0.0)
+
+
+@pytest.mark.parametrize("name", ["box_small", "mixed_bc"])
+@pytest.mark.parametrize("limiter", ["harmonic", "minmod"])
+def test_one_step_adjoint_with_body_force(mbm, name, limiter):
+    """With a body force the two stages' initial-value VJPs are taken
+    separately on the GPU (without one they are merged); both against the CPU
+    reference, which always takes them separately."""
+    one_step_vs_cpu_reference(mbm, name, limiter, 0.3)
+
+
+def one_step_vs_cpu_reference(mbm, name, limiter, force):
     import adjoint_ref as ar
     deck = mbm.read_deck(deck_path(name))
     lim = mbm.LIMITERS[limiter]
-    eng = mbm.Engine(deck.blocks, kn=0.01, limiter=limiter, scenario="tempjump")
+    eng = mbm.Engine(deck.blocks, kn=0.01, force=force, limiter=limiter, scenario="tempjump")
     eng.init()
     U0 = generic_state(mbm, eng, deck)
     mask = interior_mask(eng)
@@ -98,10 +111,10 @@ def test_one_step_adjoint_matches_cpu_reference(mbm, name, limiter):
     gI, gkn = eng.adj_backward(w)
     D = gpu_to_blocks(eng, U0)
     W = gpu_to_blocks(eng, w)
-    orc = oracle.Oracle(deck.blocks, kn=0.01, limiter=limiter, scenario="tempjump")
+    orc = oracle.Oracle(deck.blocks, kn=0.01, force=force, limiter=limiter, scenario="tempjump")
     fmap = ar.fill_transpose_map(deck.blocks)
     geoms = [deck.geom(k) for k in range(deck.nblocks)]
-    gD, gk = ar.step_adjoint(orc, deck.blocks, geoms, lim, D, W, dt, 0.01, 0.0, fmap)
+    gD, gk = ar.step_adjoint(orc, deck.blocks, geoms, lim, D, W, dt, 0.01, force, fmap)
     gD = ar.fill_T(gD, fmap)
     G = gpu_to_blocks(eng, gI)
     scale = max(np.abs(a).max() for a in gD)
