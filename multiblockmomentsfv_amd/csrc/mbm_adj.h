@@ -331,6 +331,26 @@ MBM_DEV void uf_vjp(const double *W, const double *gu, const double *gf, double 
 }
 
 /* ---------------- HLL face (frame-local) ---------------- */
+/* g . U(W) */
+MBM_DEV double dot_u(const double *W, const double *g)
+{
+  const Side sd = side_of(W);
+  double u[9], f;
+  uf<0>(W, sd, u[0], f); uf<1>(W, sd, u[1], f); uf<2>(W, sd, u[2], f);
+  uf<3>(W, sd, u[3], f); uf<4>(W, sd, u[4], f); uf<5>(W, sd, u[5], f);
+  uf<6>(W, sd, u[6], f); uf<7>(W, sd, u[7], f); uf<8>(W, sd, u[8], f);
+  double acc = 0.0;
+  for (int n = 0; n < 9; n++) acc += g[n] * u[n];
+  return acc;
+}
+
+/* Fs = ca F(WL) + cb F(WR) + cc (U(WR) - U(WL)) with cc = lamL lamR s, so the
+ * WL part is ca (F(WL) - pL U(WL)) and the WR part cb (F(WR) - pR U(WR))
+ * (pL = lamL, pR = lamR; both 0 on the alph <= 0 branch, where cc = 0). The
+ * speeds need g.F(W) and g.U(W): U and F are linear in (rho, qn, qt) at fixed
+ * velocity, theta and T, so F(W) = J_F(W) e and U(W) = J_U(W) e with
+ * e = (rho, 0, ..., 0, qn, qt), and n = J_U^T(-p g) + J_F^T g gives
+ * n . e = g.F(W) - p g.U(W) without evaluating F. */
 MBM_DEV void hll_vjp(const double *WL, const double *WR, double dtdx, const double *gF,
                      double *gWL, double *gWR)
 {
@@ -339,32 +359,29 @@ MBM_DEV void hll_vjp(const double *WL, const double *WR, double dtdx, const doub
   const double lamL = fmin(WL[1] - cL, 0.0);
   const double lamR = fmax(WR[1] + cR, 0.0);
   const double alph = lamR - lamL;
-  double ca, cb, cc, s = 0.0;
+  double ca, cb, s = 0.0, pL = 0.0, pR = 0.0;
   if (alph > 0.0) {
     s = dtdx * rcp(alph);
     ca = lamR * s;
     cb = -lamL * s;
-    cc = lamL * lamR * s;
+    pL = lamL;
+    pR = lamR;
   } else {
     ca = 0.5 * dtdx;
     cb = 0.5 * dtdx;
-    cc = 0.0;
   }
-  double uL[9], fL[9], uR[9], fR[9];
-  cons_flux(WL, uL, fL);
-  cons_flux(WR, uR, fR);
-  double guL[9], gfL[9], guR[9], gfR[9];
-  double gca = 0.0, gcb = 0.0, gcc = 0.0;
+  double mL[9], mR[9], nL[9] = { 0 }, nR[9] = { 0 };
   for (int n = 0; n < 9; n++) {
-    gfL[n] = ca * gF[n];
-    gfR[n] = cb * gF[n];
-    guR[n] = cc * gF[n];
-    guL[n] = -cc * gF[n];
-    gca += gF[n] * fL[n];
-    gcb += gF[n] * fR[n];
-    gcc += gF[n] * (uR[n] - uL[n]);
+    mL[n] = -pL * gF[n];
+    mR[n] = -pR * gF[n];
   }
+  uf_vjp(WL, mL, gF, nL);
+  uf_vjp(WR, mR, gF, nR);
   if (alph > 0.0) {
+    const double guL = dot_u(WL, gF), guR = dot_u(WR, gF);
+    const double gca = nL[0] * WL[0] + nL[7] * WL[7] + nL[8] * WL[8] + pL * guL;  /* g.F(WL) */
+    const double gcb = nR[0] * WR[0] + nR[7] * WR[7] + nR[8] * WR[8] + pR * guR;  /* g.F(WR) */
+    const double gcc = guR - guL;
     double glamR = s * gca + lamL * s * gcc;
     double glamL = -s * gcb + lamR * s * gcc;
     const double gs = lamR * gca - lamL * gcb + lamL * lamR * gcc;
@@ -381,8 +398,10 @@ MBM_DEV void hll_vjp(const double *WL, const double *WR, double dtdx, const doub
       gWR[3] += glamR * k * 0.5 * rcp(cR);
     }
   }
-  uf_vjp(WL, guL, gfL, gWL);
-  uf_vjp(WR, guR, gfR, gWR);
+  for (int n = 0; n < 9; n++) {
+    gWL[n] += ca * nL[n];
+    gWR[n] += cb * nR[n];
+  }
 }
 
 template <int DIR>

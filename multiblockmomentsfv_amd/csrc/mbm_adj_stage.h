@@ -87,22 +87,20 @@ MBM_DEV void adj_face_cell(const double *P, const double *gout, double *FxL, dou
   }
 }
 
-/* VJP of the projection of one cell (Main.c:314-336): gP is the adjoint of
- * its stage primitives, gI (may be null) that of its initial value. Adds to
+/* VJP of the projection of one cell (Main.c:314-336), given Q = prim(U), z and
+ * damp_inv(z): gP is the adjoint of its stage primitives, gI (may be null)
+ * that of its initial value. Adds to
  * gU (stage input) and, in stage 2 with DATA0, gU0 (DATA); returns the dJ/dKn
  * term. Stage 2's initial value is cons(relax(prim(U), damp3)) + the DATA
  * part cons(relax(prim(U0), damp0)); DATA0 = false leaves out the VJP of the
  * DATA part (its adjoint is gI itself; see adj_x). */
 template <int STAGE, bool DATA0 = true>
-MBM_DEV double proj_vjp(const double *U, const double *U0, const double *gP, const double *gI,
-                        double dt, double kn, double force, double *gU, double *gU0)
+MBM_DEV double proj_vjp_q(const double *U, const double *Q, double z, double i1, double i2,
+                          const double *U0, const double *gP, const double *gI, double dt,
+                          double kn, double force, double *gU, double *gU0)
 {
-  double gkn = 0.0, Q[9];
-  primitive(U, Q);
+  double gkn = 0.0;
   const double dtkn = dt / kn;
-  const double z = Q[0] * sqrtp(Q[3]) * dtkn;
-  double i1, i2;
-  damp_inv(z, i1, i2);
   double gQ[9] = { 0 }, gz = 0.0, g1 = 0.0, g2 = 0.0;
   if (STAGE == 1) {
     relax_vjp(Q, damp1(z, i1, i2), gP, gQ, g1, g2);
@@ -153,6 +151,20 @@ MBM_DEV double proj_vjp(const double *U, const double *U0, const double *gP, con
   z_vjp(Q, dtkn, kn, gz, gQ, gkn);
   primitive_vjp(U, gQ, gU);
   return gkn;
+}
+
+/* the same from U alone: Q = prim(U), z = rho sqrt(theta) dt/Kn, (i1, i2) =
+ * damp_inv(z), as the projection computes them */
+template <int STAGE, bool DATA0 = true>
+MBM_DEV double proj_vjp(const double *U, const double *U0, const double *gP, const double *gI,
+                        double dt, double kn, double force, double *gU, double *gU0)
+{
+  double Q[9];
+  primitive(U, Q);
+  const double z = Q[0] * sqrtp(Q[3]) * (dt / kn);
+  double i1, i2;
+  damp_inv(z, i1, i2);
+  return proj_vjp_q<STAGE, DATA0>(U, Q, z, i1, i2, U0, gP, gI, dt, kn, force, gU, gU0);
 }
 
 template <int STAGE, int LIM, class BLK>

@@ -137,10 +137,26 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
     const bool inner = a >= 1 && a <= lx - 2 && r >= 1 && r <= ly - 2;
     const bool ring3 = a <= 0 || a >= lx - 1 || r == 0 || r == ly - 1;
     const double w = cell_interior(a, r, lx, ly) ? A.scale : 0.0;
-    double g[NV], P[NV], dummy[NV], cp;
+    double g[NV], P[NV], Q[NV], z, i1, i2;
 #pragma unroll
     for (int n = 0; n < NV; n++) g[n] = gr[n] * w;
-    project<STAGE, false>(U, U, ring3, A.dt, dtkn, A.force, P, dummy, cp);
+    /* the projection (as project<STAGE, false>), keeping Q, z and the damping
+     * reciprocals for its VJP */
+    primitive(U, Q);
+    z = Q[0] * sqrtp(Q[3]) * dtkn;
+    damp_inv(z, i1, i2);
+    if constexpr (STAGE == 1) {
+      relax(Q, damp1(z, i1, i2), P);
+    } else {
+      Damp d3;
+      d3.d1 = i1;
+      d3.d2 = i2;
+      relax(Q, d3, P);
+    }
+    if (ring3) {
+#pragma unroll
+      for (int n = 0; n < NV; n++) P[n] = Q[n];
+    }
 
     /* x-slopes, face states; the right state of face a+1/2 comes from lane a+1 */
     double d1[NV], d2[NV], L[NV], Rn[NV], gF[NV];
@@ -195,11 +211,12 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
 #pragma unroll
           for (int n = 0; n < NV; n++) g[n] += o[n];
         }
-        if (inner) gkn += proj_vjp<1>(U, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
+        if (inner) gkn += proj_vjp_q<1>(U, Q, z, i1, i2, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
 #pragma unroll
         for (int n = 0; n < NV; n++) store1b(rgi, n * pb, boff, MERGE ? gU[n] : o[n] + gU[n]);
       } else {
-        if (inner) gkn += proj_vjp<2, !MERGE>(U, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
+        if (inner)
+          gkn += proj_vjp_q<2, !MERGE>(U, Q, z, i1, i2, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
 #pragma unroll
         for (int n = 0; n < NV; n++) {
           store1b(rgi, n * pb, boff, gU[n]);
