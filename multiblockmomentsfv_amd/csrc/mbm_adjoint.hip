@@ -49,6 +49,13 @@ struct FoldSrc {
   int pad;
 };
 
+/* one adj_y wavefront: lane l marches column a0 + l % ncols over the rows
+ * r0 + (l / ncols) * R .. + R - 1 (R = AdjArgs::yrows), clipped to the
+ * block's rows 1..ly-2, for l / ncols < ngroups */
+struct YTileDev {
+  int blk, a0, ncols, r0, ngroups, pad;
+};
+
 struct mbm_adj_state {
   int stride = 0;
   std::vector<double> dts;        /* accepted dt of every step */
@@ -58,8 +65,9 @@ struct mbm_adj_state {
   std::vector<double *> seg;      /* segment states during the reverse sweep */
   double *g[3] = { nullptr, nullptr, nullptr };   /* gN, gD, gT */
   double *gpy = nullptr;          /* y-part of gP of the current stage */
-  TileDev *xtiles = nullptr, *ytiles = nullptr;
-  int nxt = 0, nyt = 0;
+  TileDev *xtiles = nullptr;
+  YTileDev *ytiles = nullptr;
+  int nxt = 0, nyt = 0, yrows = 0;
   unsigned long long *slots = nullptr;
   double *knpart = nullptr;
   int nknpart = 0;
@@ -79,8 +87,10 @@ struct AdjArgs {
   double *gin, *gin0;
   double *gpy;                        /* y-part of the adjoint of the stage primitives */
   const BlockDev *blocks;
-  const TileDev *tiles;
+  const TileDev *tiles;               /* adj_x */
+  const YTileDev *ytiles;             /* adj_y */
   int ntiles;
+  int yrows;                          /* adj_y: rows per lane */
   double dt, kn, force, scale;
   double *knpart;                     /* one slot per tile, accumulated over launches */
 };
@@ -237,20 +247,26 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
   if (lane == 0) A.knpart[tix] += gkn;
 }
 
-/* y-direction part of S^T: march up the rows of a 64-column strip */
+/* y-direction part of S^T: every lane marches up one column over a run of
+ * rows; the lanes are independent, so a wavefront may hold 64 columns of one
+ * run or fewer columns of several runs (a block's last columns) */
 template <int STAGE, int LIM>
 __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
 {
   const int lane = threadIdx.x & 63;
   const int tix = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
   if (tix >= A.ntiles) return;
-  const TileDev T = A.tiles[tix];
-  if (T.blk < 0) return;
+  const YTileDev T = A.ytiles[tix];
   const BlockDev B = A.blocks[T.blk];
   const int lx = B.lx, ly = B.ly;
-  const int a = T.a0 + lane;
+  const int R = A.yrows;
+  const int grp = lane / T.ncols;
+  const int a = T.a0 + (lane - grp * T.ncols);
+  const int rs = T.r0 + grp * R;                 /* this lane's rows rs..re */
+  const int re = min(rs + R - 1, ly - 2);
+  const bool act = grp < T.ngroups && a <= lx - 2 && rs <= ly - 2;
   const int ac = a > lx - 1 ? lx - 1 : a;
-  const bool colin = a >= 1 && a <= lx - 2;
+  const bool colin = act && a >= 1;
   const bool colr3 = a == 0 || a >= lx - 1;
   const __amdgpu_buffer_rsrc_t rin = block_rsrc(A.in + B.off, B.plane);
   const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, B.plane);
@@ -275,25 +291,24 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
   };
 
   /* lane-private LDS rows: the stage primitives of rows k-1..k+2 (a ring
-   * indexed by row & 3), the upper face state of row k, the adjoint of its
-   * lower face state and its weighted gout. Every slot is read before it is
-   * rewritten, by the same lane. */
+   * indexed by the iteration), the upper face state of row k, the adjoint of
+   * its lower face state and its weighted gout. Every slot is read before it
+   * is rewritten, by the same lane. Iteration it handles row k = rs - 2 + it. */
   __shared__ double sm[7][NV][64];
-  auto Prow = [&](int r) -> double(&)[NV][64] { return sm[(r + 4) & 3]; };
+  auto Prow = [&](int j) -> double(&)[NV][64] { return sm[(j + 8) & 3]; };   /* row rs-2+j */
   double(&Tk)[NV][64] = sm[4];
   double(&gBk)[NV][64] = sm[5];
   double(&gk)[NV][64] = sm[6];
   /* partial gP of rows k-1 and k */
   double gPm[NV], gP0[NV];
+  const int rb = rs - 2;
 
-  /* state at k = r0-2 */
-  const int r0 = T.r0, r1 = T.r1;
-  int k = r0 - 2;
+  /* state at k = rs-2 */
   {
     double Pm[NV], Pk[NV], Pk1[NV];
-    prims(k - 1, Pm);
-    prims(k, Pk);
-    prims(k + 1, Pk1);
+    prims(rb - 1, Pm);
+    prims(rb, Pk);
+    prims(rb + 1, Pk1);
 #pragma unroll
     for (int gq = 0; gq < NV; gq += 3) {
       double d1[3], d2[3], s3[3];
@@ -307,27 +322,28 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
       for (int j = 0; j < 3; j++) Tk[gq + j][lane] = Pk[gq + j] + 0.5 * s3[j];
     }
     double g0[NV];
-    gweighted(k, g0);
+    gweighted(rb, g0);
 #pragma unroll
     for (int n = 0; n < NV; n++) {
-      Prow(k - 1)[n][lane] = Pm[n];
-      Prow(k)[n][lane] = Pk[n];
-      Prow(k + 1)[n][lane] = Pk1[n];
+      Prow(-1)[n][lane] = Pm[n];
+      Prow(0)[n][lane] = Pk[n];
+      Prow(1)[n][lane] = Pk1[n];
       gk[n][lane] = g0[n];
       gBk[n][lane] = 0.0;
       gPm[n] = gP0[n] = 0.0;
     }
   }
-  /* inputs of iteration kk (the state of row kk+2, gout of row kk+1) are
+  /* inputs of iteration it (the state of row k+2, gout of row k+1) are
    * loaded one iteration ahead, into two alternating register sets */
-  auto fetch = [&](int kk, double *U, double *g) {
-    load9b(rin, pb, col + rowb * (unsigned)rowc(kk + 2), U);
-    load9b(rg, pb, col + rowb * (unsigned)rowc(kk + 1), g);
+  auto fetch = [&](int it, double *U, double *g) {
+    load9b(rin, pb, col + rowb * (unsigned)rowc(rb + it + 2), U);
+    load9b(rg, pb, col + rowb * (unsigned)rowc(rb + it + 1), g);
   };
-  auto step = [&](int k, const double *Uc, const double *gc, double *Un, double *gn) {
-    double(&Pk)[NV][64] = Prow(k);
-    double(&Pk1)[NV][64] = Prow(k + 1);
-    double(&P2)[NV][64] = Prow(k + 2);   /* slot of row k-2, no longer read */
+  auto step = [&](int it, const double *Uc, const double *gc, double *Un, double *gn) {
+    const int k = rb + it;
+    double(&Pk)[NV][64] = Prow(it);
+    double(&Pk1)[NV][64] = Prow(it + 1);
+    double(&P2)[NV][64] = Prow(it + 2);   /* slot of row k-2, no longer read */
     /* face k+1/2: the upper state of row k (replaced below by that of row
      * k+1) and the gout difference */
     double Tl[NV], gG[NV];
@@ -365,9 +381,9 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
     }
     double gT[NV] = { 0 }, gB1[NV] = { 0 };
     face_vjp<1>(Tl, B1, dtdy, gG, gT, gB1);
-    fetch(k + 1, Un, gn);
+    fetch(it + 1, Un, gn);
     /* slope of row k: T_k = P_k + s/2, B_k = P_k - s/2 */
-    double(&Pm)[NV][64] = Prow(k - 1);
+    double(&Pm)[NV][64] = Prow(it - 1);
     double gP1[NV];
 #pragma unroll
     for (int gq = 0; gq < NV; gq += 3) {
@@ -389,7 +405,7 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
     }
     /* row k-1 is complete: its y-part goes to gpy for adj_x */
     const int rf = k - 1;
-    if (rf >= r0 && colin && rf >= 1 && rf <= ly - 2) {
+    if (colin && rf >= rs && rf <= re) {
       const unsigned boff = col + rowb * (unsigned)rf;
 #pragma unroll
       for (int n = 0; n < NV; n++) store1b(rgy, n * pb, boff, gPm[n]);
@@ -402,10 +418,11 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
     }
   };
   double Ua[NV], ga[NV], Ub[NV], gb[NV];
-  fetch(k, Ua, ga);
-  for (; k <= r1 + 1; k += 2) {
-    step(k, Ua, ga, Ub, gb);
-    if (k + 1 <= r1 + 1) step(k + 1, Ub, gb, Ua, ga);
+  const int nit = R + 3;
+  fetch(0, Ua, ga);
+  for (int it = 0; it < nit; it += 2) {
+    step(it, Ua, ga, Ub, gb);
+    if (it + 1 < nit) step(it + 1, Ub, gb, Ua, ga);
   }
 }
 
@@ -465,23 +482,35 @@ int adj_setup(mbm_ctx *c)
   HIPCHK(hipMalloc(&A->slots, (size_t)6 * c->nb * sizeof(unsigned long long)));
   HIPCHK(hipMemsetAsync(A->slots, 0, (size_t)6 * c->nb * sizeof(unsigned long long), c->stream));
   /* adj_x: 58 owned columns per wavefront (lanes 3..60), every row of the
-   * block; adj_y: 64 columns, rows 1..ly-2. Both run one wavefront per SIMD;
-   * the rows per wavefront minimise rounds x (rows + warm-up) over the
-   * device's wave slots (adj_y spends three rows of warm-up per run). */
+   * block. adj_y: one column per lane over R rows of 1..ly-2, 64 columns per
+   * wavefront, a block's remaining columns packed several row runs to a
+   * wavefront. Both run one wavefront per SIMD; the rows per wavefront
+   * minimise rounds x (rows + warm-up) over the device's wave slots (adj_y
+   * spends three rows of warm-up per run). */
   const int slots = c->cus * 4;
-  auto pick_rows = [&](int strip, int first, int lastoff, int warm) {
+  auto xcount = [&](int R) {
+    long long n = 0;
+    for (int k = 0; k < c->nb; k++) {
+      const BlockDev &B = c->hblk[k];
+      n += (long long)((B.lx + ADJX_STRIP - 1) / ADJX_STRIP) * ((B.ly + R - 1) / R);
+    }
+    return n;
+  };
+  auto ycount = [&](int R) {
+    long long n = 0;
+    for (int k = 0; k < c->nb; k++) {
+      const BlockDev &B = c->hblk[k];
+      const int cols = B.lx - 2, runs = (B.ly - 2 + R - 1) / R, rem = cols % 64;
+      n += (long long)(cols / 64) * runs;
+      if (rem) n += (runs + 64 / rem - 1) / (64 / rem);
+    }
+    return n;
+  };
+  auto pick_rows = [&](auto count, int maxrows, int warm) {
     int best = 8;
     double bestc = 1e300;
     for (int R = 4; R <= 256; R++) {
-      long long n = 0;
-      int maxrows = 0;
-      for (int k = 0; k < c->nb; k++) {
-        const BlockDev &B = c->hblk[k];
-        const int rows = B.ly - lastoff - first;
-        maxrows = std::max(maxrows, rows);
-        n += (long long)((B.lx + 3 + strip - 1) / strip) * ((rows + R - 1) / R);
-      }
-      const double cost = (double)((n + slots - 1) / slots) * (std::min(R, maxrows) + warm);
+      const double cost = (double)((count(R) + slots - 1) / slots) * (std::min(R, maxrows) + warm);
       if (cost < bestc - 1e-9) {
         bestc = cost;
         best = R;
@@ -489,23 +518,33 @@ int adj_setup(mbm_ctx *c)
     }
     return best;
   };
-  const int rx = pick_rows(ADJX_STRIP, 0, 0, 0), ry = pick_rows(64, 1, 1, 3);
-  std::vector<TileDev> xt, yt;
+  int maxly = 0;
+  for (int k = 0; k < c->nb; k++) maxly = std::max(maxly, c->hblk[k].ly);
+  const int rx = pick_rows(xcount, maxly, 0), ry = pick_rows(ycount, maxly - 2, 3);
+  std::vector<TileDev> xt;
+  std::vector<YTileDev> yt;
   for (int k = 0; k < c->nb; k++) {
     const BlockDev &B = c->hblk[k];
     for (int r0 = 0; r0 <= B.ly - 1; r0 += rx)
       for (int a0 = -3; a0 + 3 <= B.lx - 1; a0 += ADJX_STRIP)
         xt.push_back(TileDev{ k, a0, r0, std::min(r0 + rx - 1, B.ly - 1) });
-    for (int r0 = 1; r0 <= B.ly - 2; r0 += ry)
-      for (int a0 = 0; a0 <= B.lx - 2; a0 += 64)
-        yt.push_back(TileDev{ k, a0, r0, std::min(r0 + ry - 1, B.ly - 2) });
+    const int cols = B.lx - 2, runs = (B.ly - 2 + ry - 1) / ry, rem = cols % 64;
+    for (int j = 0; j < runs; j++)
+      for (int a0 = 1; a0 + 63 <= B.lx - 2; a0 += 64)
+        yt.push_back(YTileDev{ k, a0, 64, 1 + j * ry, 1, 0 });
+    if (rem) {
+      const int g = 64 / rem;
+      for (int j = 0; j < runs; j += g)
+        yt.push_back(YTileDev{ k, 1 + 64 * (cols / 64), rem, 1 + j * ry, std::min(g, runs - j), 0 });
+    }
   }
+  A->yrows = ry;
   A->nxt = (int)xt.size();
   A->nyt = (int)yt.size();
   HIPCHK(hipMalloc(&A->xtiles, xt.size() * sizeof(TileDev)));
-  HIPCHK(hipMalloc(&A->ytiles, yt.size() * sizeof(TileDev)));
+  HIPCHK(hipMalloc(&A->ytiles, yt.size() * sizeof(YTileDev)));
   HIPCHK(hipMemcpy(A->xtiles, xt.data(), xt.size() * sizeof(TileDev), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(A->ytiles, yt.data(), yt.size() * sizeof(TileDev), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(A->ytiles, yt.data(), yt.size() * sizeof(YTileDev), hipMemcpyHostToDevice));
   A->nknpart = A->nxt;
   HIPCHK(hipMalloc(&A->knpart, A->nknpart * sizeof(double)));
   /* inverse ghost map: CSR over source cells */
@@ -576,8 +615,9 @@ int stage_adjoint_lim(mbm_ctx *c, const double *in, const double *in0, const dou
   X.knpart = A->knpart;
   X.gpy = A->gpy;
   AdjArgs Y = X;
-  Y.tiles = A->ytiles;
+  Y.ytiles = A->ytiles;
   Y.ntiles = A->nyt;
+  Y.yrows = A->yrows;
   hipLaunchKernelGGL((adj_y<STAGE, LIM>), dim3(A->nyt), dim3(64), 0, c->stream, Y);
   X.tiles = A->xtiles;
   X.ntiles = A->nxt;
@@ -804,8 +844,8 @@ extern "C" int mbm_adj_release(mbm_ctx *c)
   free_checkpoints(A);
   for (double *p : { A->g[0], A->g[1], A->g[2], A->gpy })
     if (p) (void)hipFree(p);
-  for (TileDev *p : { A->xtiles, A->ytiles })
-    if (p) (void)hipFree(p);
+  if (A->xtiles) (void)hipFree(A->xtiles);
+  if (A->ytiles) (void)hipFree(A->ytiles);
   if (A->slots) (void)hipFree(A->slots);
   if (A->knpart) (void)hipFree(A->knpart);
   for (void *p : { (void *)A->fold_src, (void *)A->fold_ghost, (void *)A->fold_dir })
