@@ -11,20 +11,27 @@
  * so the sums are deterministic.
  *
  * S^T is split by direction:
- *   adj_y : one wavefront per 64 columns marching up the rows with the
- *           primitives of two rows and the partial adjoints of two rows in
- *           registers (face state, slope arguments, gout of the current row in
- *           LDS); per row: VJP of the y-face and of the y-limiter. The y-part
- *           of the adjoint of the stage primitives, gP_y, is written out;
+ *   adj_y : every lane marches up one column over a run of rows (64 columns
+ *           of one run per wavefront, or a block's last columns with several
+ *           runs packed into one wavefront), with the primitives of four rows,
+ *           the face state, its adjoint and gout of the current row in
+ *           lane-private LDS and the partial adjoints of two rows in
+ *           registers; per row: VJP of the y-face and of the y-limiter. The
+ *           y-part of the adjoint of the stage primitives, gP_y, is written
+ *           out;
  *   adj_x : row-local; one wavefront per 58 owned columns (3 halo lanes per
  *           side), lane shifts for neighbours. Per row: primitives, x-slopes,
  *           VJP of the x-faces and of the x-limiter, then the VJP of the
  *           projection of gP_x + gP_y and of the initial value, written to gin
  *           (and gin0 in stage 2). Every cell is finalised by exactly one lane.
+ * Both sweeps load row r+1 while row r is computed. Without a body force the
+ * initial value of stage 1 and the DATA part of stage 2's are the same
+ * function of DATA; their VJPs are taken once, in stage 1.
  * dJ/dKn: one partial sum per adj_x wavefront, summed on the host in a fixed
  * order. Checkpointing: DATA is stored every 'stride' steps during the
  * forward run; the reverse sweep recomputes one segment at a time and keeps
- * its DATA and stage states (memory: N/stride + 2*stride layers).
+ * its DATA and stage states (memory: N/stride + 2*stride - 1 layers, kept
+ * between runs until mbm_adj_release).
  */
 #include "mbm_internal.h"
 #include "mbm_dev.h"
@@ -85,6 +92,7 @@ constexpr int ADJ_THREADS = 256;
 struct AdjArgs {
   const double *in, *in0, *gout;
   double *gin, *gin0;
+  const double *gprev;                /* stage 1 without body force: stage 2's gout */
   double *gpy;                        /* y-part of the adjoint of the stage primitives */
   const BlockDev *blocks;
   const TileDev *tiles;               /* adj_x */
@@ -124,6 +132,8 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
   const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, B.plane);
   const __amdgpu_buffer_rsrc_t rgi = block_rsrc(A.gin + B.off, B.plane);
   const __amdgpu_buffer_rsrc_t rgi0 = block_rsrc((STAGE == 2 ? A.gin0 : A.gin) + B.off, B.plane);
+  const __amdgpu_buffer_rsrc_t rgo = block_rsrc((STAGE == 1 && MERGE ? A.gprev : A.gin) + B.off,
+                                                B.plane);
   const __amdgpu_buffer_rsrc_t rgy = block_rsrc(A.gpy + B.off, B.plane);
   const int pb = (int)(8 * B.plane);
   const unsigned rowb = 8u * (unsigned)B.pitch, col = 8u * (unsigned)ac;
@@ -133,7 +143,7 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
   auto fetch = [&](int r, double *U, double *U0, double *g) {
     const unsigned boff = col + rowb * (unsigned)r;
     load9b(rin, pb, boff, U);
-    if constexpr (STAGE == 2) load9b(rin0, pb, boff, U0);
+    if constexpr (STAGE == 2 && !MERGE) load9b(rin0, pb, boff, U0);   /* DATA part's VJP */
     load9b(rg, pb, boff, g);
   };
   /* row r from (U, U0, gr); loads row rn into (Un, U0n, gn) */
@@ -142,7 +152,7 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
     const unsigned boff = col + rowb * (unsigned)r;
     double gy[NV], o[NV];
     load9b(rgy, pb, boff, gy);
-    if constexpr (STAGE == 1) load9b(rgi, pb, boff, o);
+    if constexpr (STAGE == 1) load9b(rgo, pb, boff, o);
     fetch(rn, Un, U0n, gn);
     const bool inner = a >= 1 && a <= lx - 2 && r >= 1 && r <= ly - 2;
     const bool ring3 = a <= 0 || a >= lx - 1 || r == 0 || r == ly - 1;
@@ -213,13 +223,15 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
     for (int n = 0; n < NV; n++) gP[n] = inner ? gP[n] + gy[n] : 0.0;
     /* MERGE (no body force): stage 1's initial value and the DATA part of
      * stage 2's are the same function of DATA, so their VJPs merge: stage 2
-     * leaves its cotangent g in gin0 (= gD) and stage 1 takes g + gD. */
+     * skips its DATA part and stage 1 adds that part's cotangent, stage 2's
+     * weighted gout (1/2 gprev on interior cells), to its own. */
     if (own) {
       double gU[NV] = { 0 }, gU0[NV] = { 0 };
       if constexpr (STAGE == 1) {
         if constexpr (MERGE) {
+          const double w2 = cell_interior(a, r, lx, ly) ? 0.5 : 0.0;
 #pragma unroll
-          for (int n = 0; n < NV; n++) g[n] += o[n];
+          for (int n = 0; n < NV; n++) g[n] += o[n] * w2;
         }
         if (inner) gkn += proj_vjp_q<1>(U, Q, z, i1, i2, U0, gP, g, A.dt, A.kn, A.force, gU, gU0);
 #pragma unroll
@@ -230,7 +242,7 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
 #pragma unroll
         for (int n = 0; n < NV; n++) {
           store1b(rgi, n * pb, boff, gU[n]);
-          store1b(rgi0, n * pb, boff, MERGE ? g[n] : gU0[n]);
+          if constexpr (!MERGE) store1b(rgi0, n * pb, boff, gU0[n]);
         }
       }
     }
@@ -598,7 +610,7 @@ int fill_adjoint(mbm_ctx *c, double *g)
 
 template <int STAGE, int LIM>
 int stage_adjoint_lim(mbm_ctx *c, const double *in, const double *in0, const double *gout,
-                      double *gin, double *gin0, double dt)
+                      double *gin, double *gin0, const double *gprev, double dt)
 {
   mbm_adj_state *A = c->adj;
   AdjArgs X{};
@@ -607,6 +619,7 @@ int stage_adjoint_lim(mbm_ctx *c, const double *in, const double *in0, const dou
   X.gout = gout;
   X.gin = gin;
   X.gin0 = gin0;
+  X.gprev = gprev;
   X.blocks = c->dblk;
   X.dt = dt;
   X.kn = c->par.kn;
@@ -631,14 +644,14 @@ int stage_adjoint_lim(mbm_ctx *c, const double *in, const double *in0, const dou
 
 template <int STAGE>
 int stage_adjoint(mbm_ctx *c, const double *in, const double *in0, const double *gout,
-                  double *gin, double *gin0, double dt)
+                  double *gin, double *gin0, const double *gprev, double dt)
 {
   switch (c->par.limiter) {
-    case 0: return stage_adjoint_lim<STAGE, 0>(c, in, in0, gout, gin, gin0, dt);
-    case 1: return stage_adjoint_lim<STAGE, 1>(c, in, in0, gout, gin, gin0, dt);
-    case 2: return stage_adjoint_lim<STAGE, 2>(c, in, in0, gout, gin, gin0, dt);
-    case 3: return stage_adjoint_lim<STAGE, 3>(c, in, in0, gout, gin, gin0, dt);
-    default: return stage_adjoint_lim<STAGE, 4>(c, in, in0, gout, gin, gin0, dt);
+    case 0: return stage_adjoint_lim<STAGE, 0>(c, in, in0, gout, gin, gin0, gprev, dt);
+    case 1: return stage_adjoint_lim<STAGE, 1>(c, in, in0, gout, gin, gin0, gprev, dt);
+    case 2: return stage_adjoint_lim<STAGE, 2>(c, in, in0, gout, gin, gin0, gprev, dt);
+    case 3: return stage_adjoint_lim<STAGE, 3>(c, in, in0, gout, gin, gin0, gprev, dt);
+    default: return stage_adjoint_lim<STAGE, 4>(c, in, in0, gout, gin, gin0, gprev, dt);
   }
 }
 
@@ -669,12 +682,12 @@ int step_adjoint(mbm_ctx *c, const double *D, const double *T, double dt, double
   mbm_adj_state *A = c->adj;
   int rc;
   if ((rc = fill_adjoint(c, gN))) return rc;
-  /* stage 2 writes every cell of gT and gD; stage 1 then adds to gD (or,
-   * without a body force, takes gD as part of its initial value's cotangent
-   * and overwrites it) */
-  if ((rc = stage_adjoint<2>(c, T, D, gN, A->g[2], gD, dt))) return rc;
+  /* stage 2 writes every cell of gT and gD, stage 1 then adds to gD; without
+   * a body force stage 2 leaves gD alone and stage 1 writes every cell of it,
+   * taking the DATA part of stage 2's initial value from gN */
+  if ((rc = stage_adjoint<2>(c, T, D, gN, A->g[2], gD, nullptr, dt))) return rc;
   if ((rc = fill_adjoint(c, A->g[2]))) return rc;
-  return stage_adjoint<1>(c, D, D, A->g[2], gD, nullptr, dt);
+  return stage_adjoint<1>(c, D, D, A->g[2], gD, nullptr, gN, dt);
 }
 
 /* layer i of a pool, allocated on first use */
