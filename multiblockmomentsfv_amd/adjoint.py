@@ -141,6 +141,19 @@ class TemperatureInversion:
             v[:, 3:-3, 3:-3] = state_from_theta(th, self.rho, ex)
         return U
 
+    def final_theta(self, theta0, kn):
+        """Interior theta of every block at the end of the run (no gradient)."""
+        with torch.no_grad():
+            UN = HeunRun.apply(self.initial_state(theta0), kn, self.engine, self.dts, None)
+            return [theta(v)[3:-3, 3:-3].clone() for v in block_views(UN, self.engine)]
+
+    def objective_difference(self, th_plus, th_minus):
+        """J(+) - J(-) from two final temperature fields, summed cell by cell as
+        (th+ - th-)(th+ + th- - 2 target): the difference of two nearly equal
+        large sums is never formed."""
+        return float(sum(((p - m) * (p + m - 2 * t)).sum()
+                         for p, m, t in zip(th_plus, th_minus, self.target)))
+
     def objective(self, theta0, kn, stride=None):
         U0 = self.initial_state(theta0)
         UN = HeunRun.apply(U0, kn, self.engine, self.dts, stride)
