@@ -52,8 +52,9 @@ struct FoldGhost {
 };
 struct FoldSrc {
   long long off;
-  int plane, j0, j1;    /* ghosts j0..j1-1 copy this cell */
+  int plane, j0, j1;    /* ghosts j0..j1-1 copy this cell; j0 is inlined below */
   int pad;
+  FoldGhost first;
 };
 
 /* one adj_y wavefront: lane l marches column a0 + l % ncols over the rows
@@ -457,15 +458,20 @@ __global__ void __launch_bounds__(ADJ_THREADS) adj_fill(double *g, const FoldSrc
     return;
   }
   const FoldSrc S = src[s];
-  double acc = 0.0;
-  for (int j = S.j0; j < S.j1; j++) {
+  /* the source and its first ghost load together (ghosts are never sources) */
+  double *ps = g + S.off + (long long)m * S.plane;
+  double *p = g + S.first.off + (long long)m * S.first.plane;
+  const double vs = *ps, v = *p;
+  double acc = ((S.first.sign >> m) & 1) ? -v : v;
+  *p = 0.0;
+  for (int j = S.j0 + 1; j < S.j1; j++) {
     const FoldGhost q = ghost[j];
-    double *p = g + q.off + (long long)m * q.plane;
-    const double v = *p;
-    acc += ((q.sign >> m) & 1) ? -v : v;
-    *p = 0.0;
+    double *pq = g + q.off + (long long)m * q.plane;
+    const double w = *pq;
+    acc += ((q.sign >> m) & 1) ? -w : w;
+    *pq = 0.0;
   }
-  g[S.off + (long long)m * S.plane] += acc;
+  *ps = vs + acc;
 }
 
 /* ------------------------------------------------------------------ */
@@ -577,7 +583,8 @@ int adj_setup(mbm_ctx *c)
   for (int i : dir) fd.push_back(ghost_at(i));
   for (auto &kv : inv) {
     const BlockDev &S = c->hblk[kv.first.first];
-    FoldSrc src{ S.off + kv.first.second, (int)S.plane, (int)fg.size(), 0, 0 };
+    FoldSrc src{ S.off + kv.first.second, (int)S.plane, (int)fg.size(), 0, 0,
+                 ghost_at(kv.second[0]) };
     for (int i : kv.second) fg.push_back(ghost_at(i));
     src.j1 = (int)fg.size();
     fs.push_back(src);
