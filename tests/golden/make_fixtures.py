@@ -3,7 +3,7 @@
 Run where the reference sources exist (they are compiled in place by
 ``make -C oracle ref`` into oracle/_ref):
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py [case ...]     (default: every case)
 
 For every case below it writes the block file to tests/golden/decks/<name>.txt
 and runs ``oracle/_ref/ref_harness steplog`` (the reference's own SetBlocks,
@@ -47,6 +47,36 @@ MIXED = [(-1.0, 0.0, -1.0, 1.0, 10, 20, (-2, 2, 0, -1)),
 PERIODIC = [(-0.5, 0.5, -0.3, 0.3, 16, 12, (1, 1, -1, -1))]
 REJECT = [(-0.1, 0.1, -0.1, 0.1, 20, 20, (0, 0, 0, 0))]
 
+
+def grid_blocks(xs, ys, present, nx, ny, side_code):
+    """Blocks on the cells (i, j) of a tensor grid with present(i, j), numbered
+    row by row from the bottom; shared edges get sync codes, the others
+    side_code(i, j, side) with side 0..3 = left, right, bottom, top."""
+    ids = {}
+    for j in range(len(ys) - 1):
+        for i in range(len(xs) - 1):
+            if present(i, j):
+                ids[(i, j)] = len(ids) + 1
+    out = []
+    for (i, j), _ in sorted(ids.items(), key=lambda kv: kv[1]):
+        nb = [(i - 1, j), (i + 1, j), (i, j - 1), (i, j + 1)]
+        bc = tuple(ids.get(q, side_code(i, j, s)) if q in ids else side_code(i, j, s)
+                   for s, q in enumerate(nb))
+        out.append((xs[i], xs[i + 1], ys[j], ys[j + 1], nx, ny, bc))
+    return out
+
+
+# forward-facing step on 28 blocks of 0.5 x 1/6 (the step occupies x >= 1,
+# y < 1/3): inflow (Dirichlet) on the left, outflow on the right, walls
+# elsewhere
+FFS28 = grid_blocks([0.0, 0.5, 1.0, 1.5, 2.0, 2.5, 3.0],
+                    [0.0, 0.166667, 0.333333, 0.5, 0.666667, 0.833333, 1.0],
+                    lambda i, j: j >= 2 or i < 2, 30, 10,
+                    lambda i, j, s: -2 if s == 0 else (0 if s == 1 and i == 5 else -1))
+# 4 x 4 uniform blocks on [-1, 1]^2, walls outside
+SQUARE16 = grid_blocks([-1.0, -0.5, 0.0, 0.5, 1.0], [-1.0, -0.5, 0.0, 0.5, 1.0],
+                       lambda i, j: True, 12, 12, lambda i, j, s: -1)
+
 # name: (blocks, header parameters, accepted steps, ffs scenario)
 CASES = {
     "small_harmonic": (SQUARE_SMALL, dict(limiter="harmonic"), 30, False),
@@ -62,6 +92,8 @@ CASES = {
     "periodic": (PERIODIC, dict(limiter="harmonic", Kn=0.01), 10, False),
     "reject": (REJECT, dict(limiter="harmonic", Kn=1e-6), 5, False),
     "ffs_small": (FFS_SMALL, dict(limiter="harmonic"), 10, True),
+    "ffs28": (FFS28, dict(limiter="average"), 20, True),
+    "square16": (SQUARE16, dict(limiter="harmonic", Kn=0.01, R13=1, CFL=0.9), 20, False),
 }
 
 
@@ -82,7 +114,9 @@ def main():
     if not oracle.ref_available() or not oracle.ref_available(ffs=True):
         sys.exit("oracle/_ref is not built (make -C oracle ref)")
     os.makedirs(os.path.join(HERE, "fixtures"), exist_ok=True)
-    for name, (blocks, params, nsteps, ffs) in CASES.items():
+    names = sys.argv[1:] or list(CASES)
+    for name in names:
+        blocks, params, nsteps, ffs = CASES[name]
         path = os.path.join(HERE, "decks", name + ".txt")
         with open(path, "w") as f:
             f.write(deck_text(blocks, params))
