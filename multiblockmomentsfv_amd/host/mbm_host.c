@@ -237,9 +237,12 @@ char *mbm_convert(double a, char *s)
   int j = fabs(10 * (a - i + 1e-7));
   int k = fabs(100 * (a - i - 0.1 * j + 1e-7));
   int l = fabs(1000 * (a - i - 0.1 * j - 0.01 * k + 1e-7));
-  if (l != 0) snprintf(s, 20, "%dp%d%d%d", i, j, k, l);
-  else if (k != 0) snprintf(s, 20, "%dp%d%d", i, j, k);
-  else snprintf(s, 20, "%dp%d", i, j);
+  char t[64];   /* room for any four ints; the caller's buffer holds 20 */
+  if (l != 0) snprintf(t, sizeof t, "%dp%d%d%d", i, j, k, l);
+  else if (k != 0) snprintf(t, sizeof t, "%dp%d%d", i, j, k);
+  else snprintf(t, sizeof t, "%dp%d", i, j);
+  memcpy(s, t, 19);
+  s[19] = '\0';
   return s;
 }
 
