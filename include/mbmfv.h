@@ -150,7 +150,10 @@ int mbm_layer_set(mbm_ctx *c, int layer, const void *src);
  * No counterpart in the reference (it has no derivatives). Reverse-mode
  * derivative of a run of TimeUpdate steps with the dt schedule held fixed,
  * with respect to the state and Kn; limiter, |.| and HLL speed clipping are
- * differentiated almost everywhere. Memory: sqrt(N) checkpointing.
+ * differentiated almost everywhere. Memory: sqrt(N) checkpointing, N/stride
+ * + 2*stride - 1 layers, allocated on first use and kept (for the next
+ * forward/backward pair) until mbm_adj_release or mbm_destroy. A backward
+ * pass leaves the recorded run intact, so it may be repeated.
  *
  * mbm_adj_forward: advances DATA from its current value. dts == NULL: the
  * Steps controller of Main.c:189-244 (from DT_INITIAL, rejection, clamp to

@@ -132,6 +132,31 @@ def one_step_vs_cpu_reference(mbm, name, limiter, force):
     assert abs(gkn - gk) <= tol * abs(gk), (gkn, gk)
 
 
+def test_backward_repeats_and_release(mbm):
+    """A backward pass leaves the recorded run intact: a second one with the
+    same cotangent gives the same gradient bit for bit, also after a second
+    forward run of another length has reused the pooled layers; after
+    mbm_adj_release a backward pass is refused."""
+    deck = mbm.read_deck(deck_path("mixed_bc"))
+    eng = mbm.Engine(deck.blocks, kn=0.01, scenario="tempjump")
+    eng.init()
+    U0 = generic_state(mbm, eng, deck)
+    w = np.random.default_rng(3).normal(size=U0.size) * interior_mask(eng)
+    dts = [1e-3] * 9
+    run(eng, U0, dts, stride=3)
+    g1, k1 = eng.adj_backward(w)
+    g2, k2 = eng.adj_backward(w)
+    assert np.array_equal(g1, g2) and k1 == k2
+    run(eng, U0, [1e-3] * 16, stride=2)        # more checkpoints and segment layers
+    eng.adj_backward(w)
+    run(eng, U0, dts, stride=3)
+    g3, k3 = eng.adj_backward(w)
+    assert np.array_equal(g1, g3) and k1 == k3
+    eng.adj_release()
+    with pytest.raises(mbm.MbmError):
+        eng.adj_backward(w)
+
+
 def test_zero_step_adjoint_is_fill_transpose(mbm):
     """<fill^T g, u> == <g, fill(u)> with the GPU fill and its transpose."""
     for name in ("box_small", "lshape_small", "periodic", "mixed_bc"):
