@@ -61,3 +61,28 @@ def test_batch_tend_and_autotune(mbm):
         t, n, rej, _ = eng.steps(0.0, d["tend"], d["cfl"])
         assert n == ns[i]
         assert rel_linf(bat.download(i), eng.download(0)).max() == 0.0
+
+
+def test_full_size_sweep_subset_equals_single_runs(mbm):
+    """The 256-deck sweep at its full sizes (block sides 64..512, 88 M cells)
+    with an autotuned tile height: a few steps, then four decks against their
+    single-deck runs, bit for bit, and every deck's state finite."""
+    decks = mbm.sweep_decks(256, seed=0)
+    bat = mbm.Batch(decks)
+    bat.init()
+    bat.autotune(rows=(8, 16, 32), reps=1)
+    bat.init()
+    ns, nr = bat.run(maxsteps=6)
+    assert (ns == 6).all()
+    for i in (0, 77, 151, 255):
+        d = decks[i]
+        eng = mbm.Engine(d["blocks"], kn=d["kn"])
+        eng.init()
+        t, n, rej, _ = eng.steps(0.0, 1e30, d["cfl"], maxsteps=6)
+        assert n == 6 and rej == nr[i] and t == bat.t[i]
+        for g, r in zip(bat.download(i), eng.download(0)):
+            assert g.tobytes() == r.tobytes(), f"deck {i}"
+        eng.close()
+    for i in range(0, 256, 51):
+        assert all(np.isfinite(b).all() for b in bat.download(i))
+    bat.close()
