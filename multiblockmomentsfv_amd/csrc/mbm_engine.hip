@@ -6,11 +6,13 @@
  * Stage kernel (one Heun stage, Main.c:297-396, fused):
  *   one wavefront owns a strip of 60 columns (64 lanes: 2 halo lanes on each
  *   side) and a run of rows of one block. It marches upward row by row,
- *   keeping the primitives of rows r, r+1, the upper reconstructed state of
- *   row r, the partial update of row r and the initial value of row r+1 in
- *   registers. x-neighbours are exchanged with cross-lane moves, so there is
- *   no LDS and no barrier; every wavefront is independent. Each x-face and
- *   each y-face is evaluated once. Face speeds and the production CFL are
+ *   keeping the primitives of rows r, r+1 and the partial update of row r in
+ *   registers, and the upper reconstructed state of row r and the initial
+ *   values of rows r+1, r+2 in lane-private LDS slots; the input of row r+2
+ *   is loaded while the x-faces of row r are computed. x-neighbours are
+ *   exchanged with cross-lane moves, so there is no barrier; every wavefront
+ *   is independent. Each x-face and each y-face is evaluated once. Face
+ *   speeds and the production CFL are
  *   reduced per wavefront and folded into per-block maxima with 64-bit
  *   atomics (non-negative doubles order like their bit patterns).
  */
@@ -47,13 +49,6 @@ int set_err(int code, const char *fmt, ...)
 using namespace mbmi;
 
 using namespace mbm;
-
-/* 1: carry the upper face state and the next rows' initial values in LDS
- * (lane-private slots, 54 KiB per workgroup) instead of 36 VGPRs; with that
- * the kernel fits two waves per SIMD without spills */
-#ifndef MBM_LDS_STATE
-#define MBM_LDS_STATE 1
-#endif
 
 /* waves per SIMD the stage kernel is compiled for: 2 (256 registers per
  * wave, 220 used) lets one wave's memory waits overlap the other's FP64 work */
@@ -112,7 +107,6 @@ stage_kernel(const StageArgs A)
 
   auto row_of = [&](int r) { return r < 0 ? 0 : (r > ly - 1 ? ly - 1 : r); };
 
-#if MBM_LDS_STATE
   /* lane-private LDS slots for the state carried between rows: the upper
    * face state of the current row (slot 0) and the initial values of the next
    * two rows (slots 1, 2 by row parity); keeps them out of registers */
@@ -120,14 +114,10 @@ stage_kernel(const StageArgs A)
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 #define TUP(n) sm[wv][0][n][lane]
 #define INIT(row, n) sm[wv][1 + ((row) & 1)][n][lane]
-#endif
 
   /* ---- warm-up: rows r0-2 .. r0+1 ---- */
   const int r0 = T.r0;
   double Pa[NV], Pb[NV], Pc[NV], Pn[NV], acc[NV];
-#if !MBM_LDS_STATE
-  double In1[NV], Tup[NV];
-#endif
   {
     double Ir[NV];
     int rr = row_of(r0 - 2);
@@ -147,13 +137,7 @@ stage_kernel(const StageArgs A)
     project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp);
     if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
 #pragma unroll
-    for (int n = 0; n < NV; n++) {
-#if MBM_LDS_STATE
-      INIT(r0 + 1, n) = Ir[n];
-#else
-      In1[n] = Ir[n];
-#endif
-    }
+    for (int n = 0; n < NV; n++) INIT(r0 + 1, n) = Ir[n];
 
     double Tprev[NV], Bot[NV], G[NV], s;
     /* slopes in groups of three, exactly as in the march (same rounding) */
@@ -175,11 +159,7 @@ stage_kernel(const StageArgs A)
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         Tprev[g + k] = Pb[g + k] + 0.5 * sp[k];
-#if MBM_LDS_STATE
         TUP(g + k) = Pc[g + k] + 0.5 * sc[k];
-#else
-        Tup[g + k] = Pc[g + k] + 0.5 * sc[k];
-#endif
         Bot[g + k] = Pc[g + k] - 0.5 * sc[k];
       }
     }
@@ -189,13 +169,8 @@ stage_kernel(const StageArgs A)
     for (int n = 0; n < NV; n++) acc[n] += G[n];
   }
 
-  /* ---- march ---- */
-  for (int r = r0; r <= T.r1; r++) {
-    /* prefetch row r+2 (r+2 <= ly-1 since r <= ly-3) */
-    const unsigned roff = cin + rowb * (unsigned)(r + 2);
-    load9b(rin, pb, roff, U);
-    if constexpr (STAGE == 2) load9b(rin0, pb, roff, U0);
-
+  /* ---- march: row r, with the input of row r+2 in (Uc, U0c) ---- */
+  auto march_row = [&](int r, const double *Uc, const double *U0c) {
     /* x-faces of row r */
     {
       double L[NV], Rr[NV], F[NV], s;
@@ -226,14 +201,9 @@ stage_kernel(const StageArgs A)
     const int rr = r + 2;
     {
       double I2[NV];
-      project<STAGE, true>(U, U0, col_ring3 || rr == ly - 1, dt, dtkn, force, P2, I2, cp);
-#if MBM_LDS_STATE
+      project<STAGE, true>(Uc, U0c, col_ring3 || rr == ly - 1, dt, dtkn, force, P2, I2, cp);
 #pragma unroll
       for (int n = 0; n < NV; n++) INIT(rr, n) = I2[n];
-#else
-#pragma unroll
-      for (int n = 0; n < NV; n++) U0[n] = I2[n];    /* parked until the row ends */
-#endif
     }
     if (STAGE == 1 && col_prod && rr <= ly - 2) fold_max(mxp, cp);
     __builtin_amdgcn_sched_barrier(0);
@@ -242,13 +212,7 @@ stage_kernel(const StageArgs A)
     {
       double Tu[NV], Bn[NV], G[NV], s;
 #pragma unroll
-      for (int n = 0; n < NV; n++) {
-#if MBM_LDS_STATE
-        Tu[n] = TUP(n);
-#else
-        Tu[n] = Tup[n];
-#endif
-      }
+      for (int n = 0; n < NV; n++) Tu[n] = TUP(n);
 #pragma unroll
       for (int g = 0; g < NV; g += 3) {
         double e1[3], e2[3], sy[3];
@@ -260,11 +224,7 @@ stage_kernel(const StageArgs A)
         limit3<LIM>(e1, e2, sy);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-#if MBM_LDS_STATE
           TUP(g + k) = Pn[g + k] + 0.5 * sy[k];
-#else
-          Tup[g + k] = Pn[g + k] + 0.5 * sy[k];
-#endif
           Bn[g + k] = Pn[g + k] - 0.5 * sy[k];
         }
       }
@@ -280,21 +240,25 @@ stage_kernel(const StageArgs A)
       }
 #pragma unroll
       for (int n = 0; n < NV; n++) {
-#if MBM_LDS_STATE
         acc[n] = INIT(r + 1, n) + G[n];
-#else
-        acc[n] = In1[n] + G[n];
-        In1[n] = U0[n];
-#endif
         Pc[n] = Pn[n];
         Pn[n] = P2[n];
       }
     }
+  };
+  /* input rows (r+2 <= ly-1 since r <= ly-3; rows ahead of that clamp) */
+  auto fetch = [&](int row, double *Ud, double *U0d) {
+    const unsigned roff = cin + rowb * (unsigned)row_of(row);
+    load9b(rin, pb, roff, Ud);
+    if constexpr (STAGE == 2) load9b(rin0, pb, roff, U0d);
+  };
+  for (int r = r0; r <= T.r1; r++) {
+    /* row r+2 is loaded while the x-faces of row r are computed */
+    fetch(r + 2, U, U0);
+    march_row(r, U, U0);
   }
-#if MBM_LDS_STATE
 #undef TUP
 #undef INIT
-#endif
 
   /* ---- CFL maxima of this wavefront ---- */
   mxx = wave_max(mxx);
