@@ -256,10 +256,18 @@ def main():
     mean_launch_s = stage_ms / nstage / 1e3
     bytes_per_launch = cells * BYTES_PER_CELL_UPDATE / 2
     achieved = bytes_per_launch / mean_launch_s / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    traffic = None    # measured (rocprofv3 FETCH_SIZE + WRITE_SIZE) for the default deck only
+    if os.path.exists(TRAFFIC_FILE) and os.path.abspath(args.deck) == os.path.abspath(DECK):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    if os.path.abspath(args.deck) == os.path.abspath(DECK):
+        workload = ("square4 topology, 4 blocks x 1024x1024 (4194304 cells), Reg13, "
+                    "Kn=0.01, CFL=0.9, harmonic, Riemann2D initial state, adaptive dt")
+    else:
+        sizes = ", ".join(f"{b['nx']}x{b['ny']}" for b in deck.blocks)
+        workload = (f"{os.path.basename(args.deck)}: {deck.nblocks} blocks ({sizes}), {cells} cells, "
+                    f"Kn={deck.kn:g}, CFL={deck.cfl:g}, {deck.limiter}, Riemann2D initial state, "
+                    "adaptive dt")
     out = {
         "metric": METRIC,
         "value": value,
@@ -274,8 +282,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "square4 topology, 4 blocks x 1024x1024 (4194304 cells), Reg13, "
-                        "Kn=0.01, CFL=0.9, harmonic, Riemann2D initial state, adaptive dt",
+            "workload": workload,
             "cells": cells, "blocks": deck.nblocks, "kn": deck.kn, "r13": deck.r13,
             "cfl": deck.cfl, "limiter": deck.limiter, "t_start": round(t - 0, 6),
             "rejected_steps": rej, "tile_rows": eng.tile_rows(),
