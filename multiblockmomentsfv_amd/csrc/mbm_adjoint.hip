@@ -826,10 +826,11 @@ extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial
   HIPCHK(hipMemsetAsync(A->knpart, 0, A->nknpart * sizeof(double), c->stream));
   /* per segment: the DATA state and the stage state of every step; the
    * segment's first DATA state is its checkpoint */
-  std::vector<double *> sD(S), sT(S);
-  for (int i = 0; i < S; i++) {
+  const int L = std::min(S, std::max(N, 1));     /* steps in the longest segment */
+  std::vector<double *> sD(L), sT(L);
+  for (int i = 0; i < L; i++) {
     if (i > 0 && (rc = pool_layer(c, A->seg, i - 1, &sD[i]))) return rc;
-    if ((rc = pool_layer(c, A->seg, S - 1 + i, &sT[i]))) return rc;
+    if ((rc = pool_layer(c, A->seg, L - 1 + i, &sT[i]))) return rc;
   }
   double *gN = A->g[0], *gD = A->g[1];
   const int nseg = (N + S - 1) / S;
