@@ -80,7 +80,7 @@ def batched_leg(ndecks, steps, plain_steps, seed=0):
     decks = m.sweep_decks(ndecks, seed=seed)
     bat = m.Batch(decks)
     bat.init()
-    best, best_ms = bat.autotune(rows=(8, 16, 24, 32, 48, 64, 96), reps=2)
+    best, best_ms = bat.autotune(rows=(12, 16, 20, 24, 28, 32, 40), reps=4)
     bat.init()
     cells = np.array([sum(b["nx"] * b["ny"] for b in d["blocks"]) for d in decks], dtype=np.float64)
     torch.cuda.synchronize()
