@@ -30,8 +30,9 @@
  * dJ/dKn: one partial sum per adj_x wavefront, summed on the host in a fixed
  * order. Checkpointing: DATA is stored every 'stride' steps during the
  * forward run; the reverse sweep recomputes one segment at a time and keeps
- * its DATA and stage states (memory: N/stride + 2*stride - 1 layers, kept
- * between runs until mbm_adj_release).
+ * its DATA and stage states (memory: N/stride + 2 x (2*stride - 1) layers,
+ * kept between runs until mbm_adj_release). The recomputation of a segment
+ * runs on a second stream while the previous one is reversed.
  */
 #include "mbm_internal.h"
 #include "mbm_dev.h"
@@ -79,6 +80,10 @@ struct mbm_adj_state {
   unsigned long long *slots = nullptr;
   double *knpart = nullptr;
   int nknpart = 0;
+  /* the recomputation of the next segment runs on its own stream while the
+   * current one is reversed on the context's stream */
+  hipStream_t rstream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_ready[2] = { nullptr, nullptr }, ev_done[2] = { nullptr, nullptr };
   FoldSrc *fold_src = nullptr;     /* cells that ghosts copy, with their ghost ranges */
   FoldGhost *fold_ghost = nullptr; /* ghosts grouped by source cell */
   FoldGhost *fold_dir = nullptr;   /* Dirichlet ghosts (no source) */
@@ -601,6 +606,12 @@ int adj_setup(mbm_ctx *c)
   if ((rc = upload(&A->fold_src, fs)) || (rc = upload(&A->fold_ghost, fg)) ||
       (rc = upload(&A->fold_dir, fd)))
     return rc;
+  HIPCHK(hipStreamCreateWithFlags(&A->rstream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&A->ev_start, hipEventDisableTiming));
+  for (int q = 0; q < 2; q++) {
+    HIPCHK(hipEventCreateWithFlags(&A->ev_ready[q], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&A->ev_done[q], hipEventDisableTiming));
+  }
   return MBM_OK;
 }
 
@@ -824,26 +835,56 @@ extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial
   int rc;
   HIPCHK(hipMemcpyAsync(A->g[0], g_final, bytes, hipMemcpyDefault, c->stream));
   HIPCHK(hipMemsetAsync(A->knpart, 0, A->nknpart * sizeof(double), c->stream));
-  /* per segment: the DATA state and the stage state of every step; the
-   * segment's first DATA state is its checkpoint */
+  /* per segment: the DATA state and the stage state of every step (the
+   * segment's first DATA state is its checkpoint), in two buffer sets: the
+   * recomputation of segment s-1 (recompute stream, set (s-1)&1) runs while
+   * segment s is reversed (context stream, set s&1) */
   const int L = std::min(S, std::max(N, 1));     /* steps in the longest segment */
-  std::vector<double *> sD(L), sT(L);
-  for (int i = 0; i < L; i++) {
-    if (i > 0 && (rc = pool_layer(c, A->seg, i - 1, &sD[i]))) return rc;
-    if ((rc = pool_layer(c, A->seg, L - 1 + i, &sT[i]))) return rc;
+  std::vector<double *> sD[2], sT[2];
+  for (int q = 0; q < 2; q++) {
+    sD[q].assign(L, nullptr);
+    sT[q].assign(L, nullptr);
+    const int base = q * (2 * L - 1);
+    for (int i = 0; i < L; i++) {
+      if (i > 0 && (rc = pool_layer(c, A->seg, base + i - 1, &sD[q][i]))) return rc;
+      if ((rc = pool_layer(c, A->seg, base + L - 1 + i, &sT[q][i]))) return rc;
+    }
   }
   double *gN = A->g[0], *gD = A->g[1];
   const int nseg = (N + S - 1) / S;
+  /* the recompute stream starts after everything queued so far (the
+   * checkpoints of the forward run) */
+  HIPCHK(hipEventRecord(A->ev_start, c->stream));
+  HIPCHK(hipStreamWaitEvent(A->rstream, A->ev_start, 0));
+  auto recompute = [&](int s) -> int {
+    const int q = s & 1, k0 = s * S, k1 = std::min(N, k0 + S);
+    sD[q][0] = A->ckpt[s];
+    /* the stage and fill launchers enqueue on c->stream */
+    hipStream_t keep = c->stream;
+    c->stream = A->rstream;
+    int r = MBM_OK;
+    for (int k = k0; k < k1 - 1 && !r; k++)
+      r = step_buffers(c, sD[q][k - k0], sT[q][k - k0], sD[q][k - k0 + 1], A->dts[k]);
+    if (!r) r = stage1_buffers(c, sD[q][k1 - 1 - k0], sT[q][k1 - 1 - k0], A->dts[k1 - 1]);
+    c->stream = keep;
+    if (r) return r;
+    HIPCHK(hipEventRecord(A->ev_ready[q], A->rstream));
+    return MBM_OK;
+  };
+  if (nseg > 0 && (rc = recompute(nseg - 1))) return rc;
   for (int s = nseg - 1; s >= 0; s--) {
-    const int k0 = s * S, k1 = std::min(N, k0 + S);
-    sD[0] = A->ckpt[s];
-    for (int k = k0; k < k1 - 1; k++)
-      if ((rc = step_buffers(c, sD[k - k0], sT[k - k0], sD[k - k0 + 1], A->dts[k]))) return rc;
-    if ((rc = stage1_buffers(c, sD[k1 - 1 - k0], sT[k1 - 1 - k0], A->dts[k1 - 1]))) return rc;
+    const int q = s & 1, k0 = s * S, k1 = std::min(N, k0 + S);
+    if (s > 0) {
+      /* set (s-1)&1 was last read by the reversal of segment s+1 */
+      if (s + 1 < nseg) HIPCHK(hipStreamWaitEvent(A->rstream, A->ev_done[(s + 1) & 1], 0));
+      if ((rc = recompute(s - 1))) return rc;
+    }
+    HIPCHK(hipStreamWaitEvent(c->stream, A->ev_ready[q], 0));
     for (int k = k1 - 1; k >= k0; k--) {
-      if ((rc = step_adjoint(c, sD[k - k0], sT[k - k0], A->dts[k], gN, gD))) return rc;
+      if ((rc = step_adjoint(c, sD[q][k - k0], sT[q][k - k0], A->dts[k], gN, gD))) return rc;
       std::swap(gN, gD);
     }
+    HIPCHK(hipEventRecord(A->ev_done[q], c->stream));
   }
   if ((rc = fill_adjoint(c, gN))) return rc;
   HIPCHK(hipMemcpyAsync(g_initial, gN, bytes, hipMemcpyDefault, c->stream));
@@ -862,9 +903,13 @@ extern "C" int mbm_adj_release(mbm_ctx *c)
   if (!c || !c->adj) return MBM_OK;
   mbm_adj_state *A = c->adj;
   (void)hipStreamSynchronize(c->stream);
+  if (A->rstream) (void)hipStreamSynchronize(A->rstream);
   free_checkpoints(A);
   for (double *p : { A->g[0], A->g[1], A->g[2], A->gpy })
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : { A->ev_start, A->ev_ready[0], A->ev_ready[1], A->ev_done[0], A->ev_done[1] })
+    if (e) (void)hipEventDestroy(e);
+  if (A->rstream) (void)hipStreamDestroy(A->rstream);
   if (A->xtiles) (void)hipFree(A->xtiles);
   if (A->ytiles) (void)hipFree(A->ytiles);
   if (A->slots) (void)hipFree(A->slots);
