@@ -5,9 +5,12 @@
  * All blocks of all decks share the three layer buffers, the tile list and
  * the ghost map; every block knows its deck. Each deck has its own Steps
  * controller on the device (dt, rejection, t, DATA/NEW parity), so one Heun
- * attempt of every deck is five launches (stage 1, fill, stage 2, fill,
+ * attempt of a set of decks is five launches (stage 1, fill, stage 2, fill,
  * controller), captured once in a hipGraph and replayed without host round
- * trips. Tile height is chosen by timing candidate heights on the batch.
+ * trips. The decks form two groups of about equal size on two streams, the
+ * second half an attempt behind the first: stage 1 is bound by FP64 issue
+ * and stage 2 by memory, and one group's stage 1 runs beside the other's
+ * stage 2. Tile height is chosen by timing candidate heights on the batch.
  */
 #include "mbm_internal.h"
 
@@ -44,6 +47,32 @@ extern "C" int mbm_batch_create(const mbm_block_desc *blocks, const int *nblocks
   if (!c) return rc;
   c->batch = true;
   c->deck_parity.assign(ndecks, 0);
+  if (ndecks >= 2) {
+    /* two groups of about equal cell count, on two streams */
+    std::vector<double> cells(ndecks + 1, 0.0);
+    for (int d = 0; d < ndecks; d++) {
+      cells[d + 1] = cells[d];
+      for (int k = first[d]; k < first[d + 1]; k++) cells[d + 1] += (double)blocks[k].nx * blocks[k].ny;
+    }
+    int split = 1;
+    while (split < ndecks - 1 && cells[split] < 0.5 * cells[ndecks]) split++;
+    const std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
+    int h1 = 0;
+    while (h1 < (int)gm.size() && gm[h1].dblk < first[split]) h1++;
+    c->ngroups = 2;
+    c->grp_deck[0] = 0;
+    c->grp_deck[1] = split;
+    c->grp_deck[2] = ndecks;
+    c->grp_ghost[0] = 0;
+    c->grp_ghost[1] = h1;
+    c->grp_ghost[2] = (int)gm.size();
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->grp_ev, hipEventDisableTiming) != hipSuccess ||
+        (rc = upload_tiles(c))) {
+      mbm_destroy(c);
+      return rc ? rc : set_err(MBM_E_HIP, "batch: second stream");
+    }
+  }
   *out = c;
   return MBM_OK;
 }

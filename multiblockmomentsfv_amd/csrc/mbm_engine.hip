@@ -651,9 +651,6 @@ std::vector<TileDev> xcd_order(const std::vector<TileDev> &t)
   return o;
 }
 
-/* Tile height for a deck: every tile is one wavefront and all tiles cost the
- * same, so the launch takes ceil(ntiles / slots) rounds of one tile time
- * (rows + a warm-up of about 4 rows); slots = resident stage waves. */
 /* Tile height for a deck: every tile costs rows + 4 (warm-up), and a SIMD
  * runs w = ceil(tiles / SIMDs) waves over the launch. With two waves resident
  * per SIMD a row costs 0.87 of its one-wave time (the other wave covers the
@@ -693,7 +690,22 @@ int upload_tiles(mbm_ctx *c)
   const int rows = c->tile_rows > 0 ? c->tile_rows
                                     : auto_tile_rows(c->desc, c->cus * 4);
   c->rows_used = rows;
-  std::vector<TileDev> t = xcd_order(build_tiles(c->desc, rows));
+  std::vector<TileDev> t;
+  if (c->ngroups == 2) {
+    /* tiles come in block order: group 0's blocks first */
+    const std::vector<TileDev> all = build_tiles(c->desc, rows);
+    const int b1 = c->deck_first[c->grp_deck[1]];
+    std::vector<TileDev> t0, t1;
+    for (const TileDev &d : all) (d.blk < b1 ? t0 : t1).push_back(d);
+    t = xcd_order(t0);
+    c->grp_tile[0] = 0;
+    c->grp_tile[1] = (int)t.size();
+    const std::vector<TileDev> o1 = xcd_order(t1);
+    t.insert(t.end(), o1.begin(), o1.end());
+    c->grp_tile[2] = (int)t.size();
+  } else {
+    t = xcd_order(build_tiles(c->desc, rows));
+  }
   if (c->dtiles) HIPCHK(hipFree(c->dtiles));
   c->dtiles = nullptr;
   HIPCHK(hipMalloc(&c->dtiles, t.size() * sizeof(TileDev)));
@@ -855,39 +867,56 @@ void release_graphs(mbm_ctx *c)
         (void)hipGraphExecDestroy(c->graph[i][j]);
         c->graph_ok[i][j] = false;
       }
+  if (c->graph2_ok) {
+    (void)hipGraphExecDestroy(c->graph2);
+    c->graph2_ok = false;
+  }
 }
 
 namespace {
 
-/* one device-controlled attempt of every deck: 2 stages, 2 fills, controller */
-int enqueue_device_step(mbm_ctx *c, double *buf0, double *buf1, double *tmp)
+/* Pieces of one device-controlled attempt of group g (g < 0: every deck):
+ * stage s with the fill that follows it, and the controller. */
+int enqueue_group_stage(mbm_ctx *c, int g, int stage, double *buf0, double *buf1, double *tmp)
 {
   StageArgs A{};
+  const int t0 = g < 0 ? 0 : c->grp_tile[g], t1 = g < 0 ? c->ntiles : c->grp_tile[g + 1];
+  const int h0 = g < 0 ? 0 : c->grp_ghost[g], h1 = g < 0 ? c->nghost : c->grp_ghost[g + 1];
   A.blocks = c->dblk;
-  A.tiles = c->dtiles;
-  A.ntiles = c->ntiles;
+  A.tiles = c->dtiles + t0;
+  A.ntiles = t1 - t0;
   A.decks = c->ddecks;
   A.buf0 = buf0;
   A.buf1 = buf1;
   A.tmp = tmp;
-  A.slots = c->dslots;
+  A.slots = c->dslots + (stage == 1 ? 0 : 3 * c->nb);
   int rc;
-  if ((rc = launch_stage<1>(c, A))) return rc;
-  const int nblk = (c->nghost + 255) / 256;
-  if (c->nghost)
-    hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, (double *)nullptr, c->dblk,
-                       c->dghost, c->nghost, c->par.scenario, c->dtj, (const DeckState *)c->ddecks,
-                       buf0, buf1, tmp, 1);
-  A.slots = c->dslots + 3 * c->nb;
-  if ((rc = launch_stage<2>(c, A))) return rc;
-  if (c->nghost)
-    hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, (double *)nullptr, c->dblk,
-                       c->dghost, c->nghost, c->par.scenario, c->dtj, (const DeckState *)c->ddecks,
-                       buf0, buf1, tmp, 2);
-  hipLaunchKernelGGL(control_kernel, dim3((c->ndecks + 63) / 64), dim3(64), 0, c->stream, c->ddecks,
-                     c->dslots, c->dblk, c->ddeck_first, c->ndecks, c->nb, c->drec);
+  if ((rc = stage == 1 ? launch_stage<1>(c, A) : launch_stage<2>(c, A))) return rc;
+  if (h1 > h0)
+    hipLaunchKernelGGL(fill_kernel, dim3((h1 - h0 + 255) / 256), dim3(256), 0, c->stream,
+                       (double *)nullptr, c->dblk, c->dghost + h0, h1 - h0, c->par.scenario, c->dtj,
+                       (const DeckState *)c->ddecks, buf0, buf1, tmp, stage);
   HIPCHK(hipGetLastError());
   return MBM_OK;
+}
+
+int enqueue_group_control(mbm_ctx *c, int g)
+{
+  const int d0 = g < 0 ? 0 : c->grp_deck[g], d1 = g < 0 ? c->ndecks : c->grp_deck[g + 1];
+  hipLaunchKernelGGL(control_kernel, dim3((d1 - d0 + 63) / 64), dim3(64), 0, c->stream,
+                     c->ddecks + d0, c->dslots, c->dblk, c->ddeck_first + d0, d1 - d0, c->nb,
+                     g > 0 ? (RecordDev *)nullptr : c->drec);
+  HIPCHK(hipGetLastError());
+  return MBM_OK;
+}
+
+/* one device-controlled attempt of every deck: 2 stages, 2 fills, controller */
+int enqueue_device_step(mbm_ctx *c, double *buf0, double *buf1, double *tmp)
+{
+  int rc;
+  if ((rc = enqueue_group_stage(c, -1, 1, buf0, buf1, tmp))) return rc;
+  if ((rc = enqueue_group_stage(c, -1, 2, buf0, buf1, tmp))) return rc;
+  return enqueue_group_control(c, -1);
 }
 
 } // namespace
@@ -925,25 +954,66 @@ int clear_slots(mbm_ctx *c)
  * The current DATA/NEW roles are buf0/buf1 at parity 0; on return the roles
  * of the physical buffers are updated from deck 0's parity (a plain context
  * has one deck; batches keep per-deck parity in ddecks). */
+/* capture what 'enq' enqueues on stream s into an executable graph */
+template <class F>
+int capture(mbm_ctx *c, hipStream_t s, hipGraphExec_t *out, F enq)
+{
+  hipGraph_t g;
+  hipStream_t keep = c->stream;
+  c->stream = s;                  /* the enqueue helpers launch on c->stream */
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = enq();
+  hipError_t e = hipStreamEndCapture(s, &g);
+  c->stream = keep;
+  if (rc) return rc;
+  if (e != hipSuccess) return set_err(MBM_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return set_err(MBM_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+  return MBM_OK;
+}
+
 int device_steps(mbm_ctx *c, int maxchunk)
 {
   const int iD = c->role[MBM_L_DATA], iN = c->role[MBM_L_NEW], iT = c->role[MBM_L_TMP];
   double *buf0 = c->layer[iD], *buf1 = c->layer[iN], *tmp = c->layer[iT];
+  const bool two = c->ngroups == 2;
+  int rc;
   if (!c->graph_ok[iD][iN]) {
-    hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_device_step(c, buf0, buf1, tmp);
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (two)   /* group 0: stage 1, stage 2, controller */
+      rc = capture(c, c->stream, &c->graph[iD][iN], [&] {
+        int r = enqueue_group_stage(c, 0, 1, buf0, buf1, tmp);
+        if (!r) r = enqueue_group_stage(c, 0, 2, buf0, buf1, tmp);
+        return r ? r : enqueue_group_control(c, 0);
+      });
+    else
+      rc = capture(c, c->stream, &c->graph[iD][iN],
+                   [&] { return enqueue_device_step(c, buf0, buf1, tmp); });
     if (rc) return rc;
-    if (e != hipSuccess) return set_err(MBM_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&c->graph[iD][iN], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return set_err(MBM_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
     c->graph_ok[iD][iN] = true;
+  }
+  if (two && !c->graph2_ok) {
+    /* group 1, half an attempt later: stage 2, controller, next stage 1 */
+    rc = capture(c, c->stream2, &c->graph2, [&] {
+      int r = enqueue_group_stage(c, 1, 2, buf0, buf1, tmp);
+      if (!r) r = enqueue_group_control(c, 1);
+      return r ? r : enqueue_group_stage(c, 1, 1, buf0, buf1, tmp);
+    });
+    if (rc) return rc;
+    c->graph2_ok = true;
   }
   std::vector<DeckState> st(c->ndecks);
   HIPCHK(hipMemcpyAsync(st.data(), c->ddecks, c->ndecks * sizeof(DeckState), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (two) {
+    /* group 1's first stage 1 (the stream is idle: the host has synchronised
+     * with everything queued on c->stream) */
+    hipStream_t keep = c->stream;
+    c->stream = c->stream2;
+    rc = enqueue_group_stage(c, 1, 1, buf0, buf1, tmp);
+    c->stream = keep;
+    if (rc) return rc;
+  }
   for (;;) {
     int remaining = 0;
     for (const DeckState &S : st)
@@ -956,7 +1026,14 @@ int device_steps(mbm_ctx *c, int maxchunk)
       }
     if (remaining == 0) break;
     const int chunk = std::min(remaining, maxchunk);
-    for (int i = 0; i < chunk; i++) HIPCHK(hipGraphLaunch(c->graph[iD][iN], c->stream));
+    for (int i = 0; i < chunk; i++) {
+      HIPCHK(hipGraphLaunch(c->graph[iD][iN], c->stream));
+      if (two) HIPCHK(hipGraphLaunch(c->graph2, c->stream2));
+    }
+    if (two) {
+      HIPCHK(hipEventRecord(c->grp_ev, c->stream2));
+      HIPCHK(hipStreamWaitEvent(c->stream, c->grp_ev, 0));
+    }
     HIPCHK(hipMemcpyAsync(st.data(), c->ddecks, c->ndecks * sizeof(DeckState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
@@ -984,6 +1061,7 @@ extern "C" int mbm_destroy(mbm_ctx *c)
 {
   if (!c) return MBM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   if (c->adj) mbm_adj_release(c);
   release_graphs(c);
   for (int l = 0; l < 3; l++) if (c->layer[l]) (void)hipFree(c->layer[l]);
@@ -998,6 +1076,8 @@ extern "C" int mbm_destroy(mbm_ctx *c)
   if (c->ddecks) (void)hipFree(c->ddecks);
   if (c->drec) (void)hipFree(c->drec);
   for (int i = 0; i < 4; i++) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  if (c->grp_ev) (void)hipEventDestroy(c->grp_ev);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MBM_OK;

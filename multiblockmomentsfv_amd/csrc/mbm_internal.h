@@ -136,6 +136,16 @@ struct mbm_ctx {
   /* captured Heun step + controller, per (DATA, NEW) buffer assignment */
   hipGraphExec_t graph[3][3];
   bool graph_ok[3][3];
+  /* batches of two or more decks: decks [grp_deck[g], grp_deck[g+1]) form
+   * group g (g = 0, 1), with tiles [grp_tile[g], grp_tile[g+1]) and ghosts
+   * [grp_ghost[g], grp_ghost[g+1]). Group 1 runs on stream2, its attempt
+   * shifted by half (stage 2 of one group overlaps stage 1 of the other). */
+  int ngroups;                    /* 0: one group (plain contexts) */
+  int grp_deck[3], grp_tile[3], grp_ghost[3];
+  hipStream_t stream2;
+  hipEvent_t grp_ev;
+  hipGraphExec_t graph2;
+  bool graph2_ok;
 };
 
 namespace mbmi {
