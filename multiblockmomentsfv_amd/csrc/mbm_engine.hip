@@ -275,36 +275,35 @@ stage_kernel(const StageArgs A)
 /* SetBoundary as a gather over the precomputed ghost map (Boundary.c:173-193).
  * decks != nullptr: layer chosen per deck (which 1: TMP, 2: NEW by parity),
  * decks that are done are skipped. */
-__global__ void __launch_bounds__(256) fill_kernel(double *layer, const BlockDev *blocks,
-                                                   const GhostDev *map, int n, int scenario,
+__global__ void __launch_bounds__(256) fill_kernel(double *layer, const FillRec *rec,
+                                                   const double2 *xy, int n, int scenario,
                                                    const double *tj, const DeckState *decks,
                                                    double *buf0, double *buf1, double *tmp,
                                                    int which)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const GhostDev g = map[i];
-  const BlockDev D = blocks[g.dblk];
+  const FillRec g = rec[i];
   if (decks) {
-    const DeckState &S = decks[D.deck];
+    const DeckState &S = decks[g.deck];
     if (S.done) return;
     layer = which == 1 ? tmp : (S.parity ? buf0 : buf1);
   }
   double v[NV];
-  if (g.sblk >= 0) {
-    const BlockDev S = blocks[g.sblk];
-    const double *src = layer + S.off + g.scell;
+  if (g.src >= 0) {
+    const double *src = layer + g.src;
 #pragma unroll
-    for (int m = 0; m < NV; m++) v[m] = src[m * S.plane];
+    for (int m = 0; m < NV; m++) v[m] = src[(long long)m * g.splane];
   } else {
     double tjl[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) tjl[m] = tj[m];
-    mbm_scn_eval(scenario, tjl, g.x, g.y, v);
+    const double2 p = xy[i];
+    mbm_scn_eval(scenario, tjl, p.x, p.y, v);
   }
-  double *dst = layer + D.off + g.dcell;
+  double *dst = layer + g.dst;
 #pragma unroll
-  for (int m = 0; m < NV; m++) dst[m * D.plane] = ((g.sign >> m) & 1) ? -v[m] : v[m];
+  for (int m = 0; m < NV; m++) dst[(long long)m * g.dplane] = ((g.sign >> m) & 1) ? -v[m] : v[m];
 }
 
 /* Steps controller of every deck (Main.c:202-216) after one TimeUpdate:
@@ -733,8 +732,8 @@ int fill_buffer(mbm_ctx *c, double *layer)
 {
   if (c->nghost == 0) return MBM_OK;
   const int nblk = (c->nghost + 255) / 256;
-  hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, layer, c->dblk, c->dghost,
-                     c->nghost, c->par.scenario, c->dtj, (const DeckState *)nullptr,
+  hipLaunchKernelGGL(fill_kernel, dim3(nblk), dim3(256), 0, c->stream, layer, c->dfill,
+                     c->dfillxy, c->nghost, c->par.scenario, c->dtj, (const DeckState *)nullptr,
                      (double *)nullptr, (double *)nullptr, (double *)nullptr, 0);
   HIPCHK(hipGetLastError());
   return MBM_OK;
@@ -838,11 +837,26 @@ mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_f
   }
   std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
   c->nghost = (int)gm.size();
+  std::vector<FillRec> fr(gm.size());
+  std::vector<double2> fxy(gm.size());
+  for (size_t i = 0; i < gm.size(); i++) {
+    const GhostDev &g = gm[i];
+    const BlockDev &D = c->hblk[g.dblk];
+    fr[i].dst = D.off + g.dcell;
+    fr[i].dplane = (int)D.plane;
+    fr[i].src = g.sblk >= 0 ? c->hblk[g.sblk].off + g.scell : -1;
+    fr[i].splane = g.sblk >= 0 ? (int)c->hblk[g.sblk].plane : 0;
+    fr[i].deck = D.deck;
+    fr[i].sign = g.sign;
+    fxy[i] = make_double2(g.x, g.y);
+  }
   const bool ok =
       hipok(hipMalloc(&c->dblk, nblocks * sizeof(BlockDev)), "hipMalloc") &&
       hipok(hipMemcpy(c->dblk, c->hblk.data(), nblocks * sizeof(BlockDev), hipMemcpyHostToDevice), "hipMemcpy") &&
-      hipok(hipMalloc(&c->dghost, std::max<size_t>(1, gm.size()) * sizeof(GhostDev)), "hipMalloc") &&
-      (gm.empty() || hipok(hipMemcpy(c->dghost, gm.data(), gm.size() * sizeof(GhostDev), hipMemcpyHostToDevice), "hipMemcpy")) &&
+      hipok(hipMalloc(&c->dfill, std::max<size_t>(1, fr.size()) * sizeof(FillRec)), "hipMalloc") &&
+      (fr.empty() || hipok(hipMemcpy(c->dfill, fr.data(), fr.size() * sizeof(FillRec), hipMemcpyHostToDevice), "hipMemcpy")) &&
+      hipok(hipMalloc(&c->dfillxy, std::max<size_t>(1, fxy.size()) * sizeof(double2)), "hipMalloc") &&
+      (fxy.empty() || hipok(hipMemcpy(c->dfillxy, fxy.data(), fxy.size() * sizeof(double2), hipMemcpyHostToDevice), "hipMemcpy")) &&
       hipok(hipMalloc(&c->dslots, (size_t)6 * nblocks * sizeof(unsigned long long)), "hipMalloc") &&
       hipok(hipMemsetAsync(c->dslots, 0, (size_t)6 * nblocks * sizeof(unsigned long long), c->stream), "hipMemsetAsync") &&
       hipok(hipMalloc(&c->dcfl, sizeof(double)), "hipMalloc") &&
@@ -894,7 +908,7 @@ int enqueue_group_stage(mbm_ctx *c, int g, int stage, double *buf0, double *buf1
   if ((rc = stage == 1 ? launch_stage<1>(c, A) : launch_stage<2>(c, A))) return rc;
   if (h1 > h0)
     hipLaunchKernelGGL(fill_kernel, dim3((h1 - h0 + 255) / 256), dim3(256), 0, c->stream,
-                       (double *)nullptr, c->dblk, c->dghost + h0, h1 - h0, c->par.scenario, c->dtj,
+                       (double *)nullptr, c->dfill + h0, c->dfillxy + h0, h1 - h0, c->par.scenario, c->dtj,
                        (const DeckState *)c->ddecks, buf0, buf1, tmp, stage);
   HIPCHK(hipGetLastError());
   return MBM_OK;
@@ -1067,7 +1081,8 @@ extern "C" int mbm_destroy(mbm_ctx *c)
   for (int l = 0; l < 3; l++) if (c->layer[l]) (void)hipFree(c->layer[l]);
   if (c->dblk) (void)hipFree(c->dblk);
   if (c->dtiles) (void)hipFree(c->dtiles);
-  if (c->dghost) (void)hipFree(c->dghost);
+  if (c->dfill) (void)hipFree(c->dfill);
+  if (c->dfillxy) (void)hipFree(c->dfillxy);
   if (c->dslots) (void)hipFree(c->dslots);
   if (c->dcfl) (void)hipFree(c->dcfl);
   if (c->dtj) (void)hipFree(c->dtj);

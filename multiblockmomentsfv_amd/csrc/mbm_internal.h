@@ -70,6 +70,14 @@ struct GhostDev {
   double x, y;
 };
 
+/* device form of one ghost for the fill: element offsets of component 0 in
+ * a layer (src < 0: the scenario state at the ghost's (x, y)) */
+struct FillRec {
+  long long dst, src;
+  int dplane, splane;
+  int deck, sign;     /* sign: bit m set, component m negated */
+};
+
 struct StageArgs {
   const double *in;   /* stage 1: DATA, stage 2: TMP */
   const double *in0;  /* stage 2: DATA */
@@ -94,6 +102,7 @@ using mbmi::TileDev;
 using mbmi::GhostDev;
 using mbmi::DeckState;
 using mbmi::RecordDev;
+using mbmi::FillRec;
 
 struct mbm_ctx {
   int nb;
@@ -109,7 +118,8 @@ struct mbm_ctx {
   int tile_rows;             /* requested tile height, 0 = automatic */
   int rows_used;             /* tile height of the current tile list */
   int cus;                   /* compute units of the device */
-  GhostDev *dghost;
+  FillRec *dfill;             /* ghost fill records, in block order */
+  double2 *dfillxy;           /* ghost cell centres (scenario ghosts) */
   int nghost;
   unsigned long long *dslots; /* 2 stages x nb x 3 */
   double *dcfl;               /* combined CFL */
