@@ -31,8 +31,8 @@
  * order. Checkpointing: DATA is stored every 'stride' steps during the
  * forward run; the reverse sweep recomputes one segment at a time and keeps
  * its DATA and stage states (memory: N/stride + 2 x (2*stride - 1) layers,
- * kept between runs until mbm_adj_release). The recomputation of a segment
- * runs on a second stream while the previous one is reversed.
+ * kept between runs until mbm_adj_release). Segment s-1 is recomputed on a
+ * second stream while segment s is reversed.
  */
 #include "mbm_internal.h"
 #include "mbm_dev.h"
