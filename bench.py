@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--deck PATH] [--no-cpu]
                     [--no-batched] [--no-adjoint]
+    python bench.py --calibration-copy | --stream-rates
 
 Workload: square4 topology with every block 1024x1024 (4 194 304 cells),
 Reg13, Kn=0.01, CFL=0.9, harmonic limiter, Riemann2D initial state, FP64,
@@ -201,6 +202,46 @@ def adjoint_leg(deck_path, fd_dirs, tend=0.3):
                       "cell by cell; timings: the faster of two passes after one untimed pass"}
 
 
+def calibration_copy():
+    """Five plain device-to-device copies of 256 MiB: the reference dispatches
+    against which rocprofv3's FETCH_SIZE / WRITE_SIZE are scaled to bytes."""
+    import torch
+    x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": x.numel() * 8, "copies": 5}))
+
+
+def stream_rates():
+    """Streaming HBM rates of plain torch kernels: a copy (one read, one write)
+    and an add (two reads, one write, the byte mix of stage 2), at the size of
+    one 4x1024^2 layer (305 MB per array) and at 2 GiB per array (no reuse
+    through the 256 MB memory-side cache)."""
+    import torch
+    out = {}
+    for label, n in (("305MB", 4243600 * 9), ("2GiB", 256 * 1024 * 1024)):
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.ones_like(x)
+        z = torch.empty_like(x)
+        for name, fn, arrays in (("copy", lambda: z.copy_(x), 2), ("add", lambda: torch.add(x, y, out=z), 3)):
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            for _ in range(20):
+                fn()
+            ev[1].record()
+            torch.cuda.synchronize()
+            sec = ev[0].elapsed_time(ev[1]) / 1e3 / 20
+            out[f"{name}_{label}_TBps"] = arrays * n * 8 / sec / 1e12
+        del x, y, z
+        torch.cuda.empty_cache()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,7 +257,17 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=50)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fd-dirs", type=int, default=32)
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="only run the 256 MiB reference copies for the PMC calibration")
+    ap.add_argument("--stream-rates", action="store_true",
+                    help="only measure the streaming copy/add rates")
     args = ap.parse_args()
+    if args.calibration_copy:
+        calibration_copy()
+        return
+    if args.stream_rates:
+        stream_rates()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design note, not built)", file=sys.stderr)

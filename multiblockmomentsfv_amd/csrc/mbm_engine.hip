@@ -5,11 +5,12 @@
  *
  * Stage kernel (one Heun stage, Main.c:297-396, fused):
  *   one wavefront owns a strip of 60 columns (64 lanes: 2 halo lanes on each
- *   side) and a run of rows of one block. It marches upward row by row,
- *   keeping the primitives of rows r, r+1 and the partial update of row r in
- *   registers, and the upper reconstructed state of row r and the initial
- *   values of rows r+1, r+2 in lane-private LDS slots; the input of row r+2
- *   is loaded while the x-faces of row r are computed. x-neighbours are
+ *   side) and a run of rows of one block. It marches row by row (stage 1
+ *   upward, stage 2 downward), keeping the primitives of rows r, r+-1 and the
+ *   partial update of row r in registers, and the reconstructed state of row
+ *   r on the face ahead and the initial values of the next two rows in
+ *   lane-private LDS slots; the input of row r+-2 is loaded while the x-faces
+ *   of row r are computed. x-neighbours are
  *   exchanged with cross-lane moves, so there is no barrier; every wavefront
  *   is independent. Each x-face and each y-face is evaluated once. Face
  *   speeds and the production CFL are
@@ -56,10 +57,19 @@ using namespace mbm;
 #define MBM_STAGE_WAVES 2
 #endif
 
-template <int STAGE, int LIM>
+/* stage 2 marches its tiles downward (MBM_STAGE2_DOWN=1): stage 1 ends at
+ * the top rows of every tile, whose input and output are then the most
+ * recently touched lines in the memory-side cache, and stage 2 starts there */
+#ifndef MBM_STAGE2_DOWN
+#define MBM_STAGE2_DOWN 1
+#endif
+
+template <int STAGE, int LIM, bool DOWN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MBM_STAGE_WAVES, MBM_STAGE_WAVES)))
 stage_kernel(const StageArgs A)
 {
+  /* the production CFL (stage 1) is folded by the upward march only */
+  static_assert(!DOWN || STAGE == 2, "downward march is stage 2 only");
   const int lane = threadIdx.x & 63;
   /* wave-uniform: tile, block and deck data are loaded into scalar registers */
   const int tix = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6));
@@ -107,155 +117,275 @@ stage_kernel(const StageArgs A)
 
   auto row_of = [&](int r) { return r < 0 ? 0 : (r > ly - 1 ? ly - 1 : r); };
 
-  /* lane-private LDS slots for the state carried between rows: the upper
-   * face state of the current row (slot 0) and the initial values of the next
-   * two rows (slots 1, 2 by row parity); keeps them out of registers */
+  /* lane-private LDS slots for the state carried between rows: the
+   * reconstructed state of the current row on the face ahead of the march
+   * (slot 0: upper state when marching up, lower state when marching down)
+   * and the initial values of the next two rows (slots 1, 2 by row parity);
+   * keeps them out of registers */
   __shared__ double sm[WAVES_PER_WG][3][NV][64];
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 #define TUP(n) sm[wv][0][n][lane]
 #define INIT(row, n) sm[wv][1 + ((row) & 1)][n][lane]
 
-  /* ---- warm-up: rows r0-2 .. r0+1 ---- */
-  const int r0 = T.r0;
   double Pa[NV], Pb[NV], Pc[NV], Pn[NV], acc[NV];
-  {
-    double Ir[NV];
-    int rr = row_of(r0 - 2);
-    load9b(rin, pb, cin + rowb * rr, U);
-    project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
-    rr = row_of(r0 - 1);
-    load9b(rin, pb, cin + rowb * rr, U);
-    project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp);
-    rr = row_of(r0);
-    load9b(rin, pb, cin + rowb * rr, U);
-    if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb * rr, U0);
-    project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp);
-    if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
-    rr = row_of(r0 + 1);
-    load9b(rin, pb, cin + rowb * rr, U);
-    if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb * rr, U0);
-    project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp);
-    if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
-#pragma unroll
-    for (int n = 0; n < NV; n++) INIT(r0 + 1, n) = Ir[n];
 
-    double Tprev[NV], Bot[NV], G[NV], s;
-    /* slopes in groups of three, exactly as in the march (same rounding) */
+  /* x-faces of the current row (primitives in Pc): acc += F(i-1/2) - F(i+1/2) */
+  auto xfaces = [&]() {
+    double L[NV], Rr[NV], F[NV], s;
 #pragma unroll
     for (int g = 0; g < NV; g += 3) {
-      double e1[3], e2[3], sp[3], sc[3];
+      double e1[3], e2[3], sx[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        e1[k] = Pb[g + k] - Pa[g + k];
-        e2[k] = Pc[g + k] - Pb[g + k];
+        e1[k] = Pc[g + k] - from_left(Pc[g + k]);
+        e2[k] = from_right(Pc[g + k]) - Pc[g + k];
       }
-      limit3<LIM>(e1, e2, sp);
+      limit3<LIM>(e1, e2, sx);
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        e1[k] = Pc[g + k] - Pb[g + k];
-        e2[k] = Pn[g + k] - Pc[g + k];
-      }
-      limit3<LIM>(e1, e2, sc);
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        Tprev[g + k] = Pb[g + k] + 0.5 * sp[k];
-        TUP(g + k) = Pc[g + k] + 0.5 * sc[k];
-        Bot[g + k] = Pc[g + k] - 0.5 * sc[k];
+        L[g + k] = Pc[g + k] + 0.5 * sx[k];
+        Rr[g + k] = from_right(Pc[g + k] - 0.5 * sx[k]);
       }
     }
-    face<1>(Tprev, Bot, dtdy, G, s);
-    if (col_face && r0 - 1 >= 1 && r0 - 1 <= ly - 3) fold_max(mxy, s);
+    face<0>(L, Rr, dtdx, F, s);
+    if (xface_ok) fold_max(mxx, s);
 #pragma unroll
-    for (int n = 0; n < NV; n++) acc[n] += G[n];
-  }
-
-  /* ---- march: row r, with the input of row r+2 in (Uc, U0c) ---- */
-  auto march_row = [&](int r, const double *Uc, const double *U0c) {
-    /* x-faces of row r */
-    {
-      double L[NV], Rr[NV], F[NV], s;
-#pragma unroll
-      for (int g = 0; g < NV; g += 3) {
-        double e1[3], e2[3], sx[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          e1[k] = Pc[g + k] - from_left(Pc[g + k]);
-          e2[k] = from_right(Pc[g + k]) - Pc[g + k];
-        }
-        limit3<LIM>(e1, e2, sx);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          L[g + k] = Pc[g + k] + 0.5 * sx[k];
-          Rr[g + k] = from_right(Pc[g + k] - 0.5 * sx[k]);
-        }
-      }
-      face<0>(L, Rr, dtdx, F, s);
-      if (xface_ok) fold_max(mxx, s);
-#pragma unroll
-      for (int n = 0; n < NV; n++) acc[n] += from_left(F[n]) - F[n];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-
-    /* row r+2: projection */
-    double P2[NV];
-    const int rr = r + 2;
-    {
-      double I2[NV];
-      project<STAGE, true>(Uc, U0c, col_ring3 || rr == ly - 1, dt, dtkn, force, P2, I2, cp);
-#pragma unroll
-      for (int n = 0; n < NV; n++) INIT(rr, n) = I2[n];
-    }
-    if (STAGE == 1 && col_prod && rr <= ly - 2) fold_max(mxp, cp);
-    __builtin_amdgcn_sched_barrier(0);
-
-    /* y-face r+1/2 */
-    {
-      double Tu[NV], Bn[NV], G[NV], s;
-#pragma unroll
-      for (int n = 0; n < NV; n++) Tu[n] = TUP(n);
-#pragma unroll
-      for (int g = 0; g < NV; g += 3) {
-        double e1[3], e2[3], sy[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          e1[k] = Pn[g + k] - Pc[g + k];
-          e2[k] = P2[g + k] - Pn[g + k];
-        }
-        limit3<LIM>(e1, e2, sy);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          TUP(g + k) = Pn[g + k] + 0.5 * sy[k];
-          Bn[g + k] = Pn[g + k] - 0.5 * sy[k];
-        }
-      }
-      face<1>(Tu, Bn, dtdy, G, s);
-      if (col_face) fold_max(mxy, s);
-      if (st_col && r >= MBC && r <= ly - 1 - MBC) {
-        const unsigned ob = cout + rowb * (unsigned)r;
-#pragma unroll
-        for (int n = 0; n < NV; n++) {
-          const double v = acc[n] - G[n];
-          store1b(rout, n * pb, ob, (STAGE == 2) ? 0.5 * v : v);
-        }
-      }
-#pragma unroll
-      for (int n = 0; n < NV; n++) {
-        acc[n] = INIT(r + 1, n) + G[n];
-        Pc[n] = Pn[n];
-        Pn[n] = P2[n];
-      }
-    }
+    for (int n = 0; n < NV; n++) acc[n] += from_left(F[n]) - F[n];
   };
-  /* input rows (r+2 <= ly-1 since r <= ly-3; rows ahead of that clamp) */
+  /* input rows, clamped to the block (row 0 and ly-1 are ghost ring 3) */
   auto fetch = [&](int row, double *Ud, double *U0d) {
     const unsigned roff = cin + rowb * (unsigned)row_of(row);
     load9b(rin, pb, roff, Ud);
     if constexpr (STAGE == 2) load9b(rin0, pb, roff, U0d);
   };
-  for (int r = r0; r <= T.r1; r++) {
-    /* row r+2 is loaded while the x-faces of row r are computed */
-    fetch(r + 2, U, U0);
-    march_row(r, U, U0);
+
+  if constexpr (!DOWN) {
+    /* ---- warm-up: rows r0-2 .. r0+1 ---- */
+    const int r0 = T.r0;
+    {
+      double Ir[NV];
+      int rr = row_of(r0 - 2);
+      load9b(rin, pb, cin + rowb * rr, U);
+      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
+      rr = row_of(r0 - 1);
+      load9b(rin, pb, cin + rowb * rr, U);
+      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp);
+      rr = row_of(r0);
+      load9b(rin, pb, cin + rowb * rr, U);
+      if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb * rr, U0);
+      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp);
+      if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
+      rr = row_of(r0 + 1);
+      load9b(rin, pb, cin + rowb * rr, U);
+      if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb * rr, U0);
+      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp);
+      if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
+#pragma unroll
+      for (int n = 0; n < NV; n++) INIT(r0 + 1, n) = Ir[n];
+
+      double Tprev[NV], Bot[NV], G[NV], s;
+      /* slopes in groups of three, exactly as in the march (same rounding) */
+#pragma unroll
+      for (int g = 0; g < NV; g += 3) {
+        double e1[3], e2[3], sp[3], sc[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          e1[k] = Pb[g + k] - Pa[g + k];
+          e2[k] = Pc[g + k] - Pb[g + k];
+        }
+        limit3<LIM>(e1, e2, sp);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          e1[k] = Pc[g + k] - Pb[g + k];
+          e2[k] = Pn[g + k] - Pc[g + k];
+        }
+        limit3<LIM>(e1, e2, sc);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          Tprev[g + k] = Pb[g + k] + 0.5 * sp[k];
+          TUP(g + k) = Pc[g + k] + 0.5 * sc[k];
+          Bot[g + k] = Pc[g + k] - 0.5 * sc[k];
+        }
+      }
+      face<1>(Tprev, Bot, dtdy, G, s);
+      if (col_face && r0 - 1 >= 1 && r0 - 1 <= ly - 3) fold_max(mxy, s);
+#pragma unroll
+      for (int n = 0; n < NV; n++) acc[n] += G[n];
+    }
+
+    /* ---- march: row r, with the input of row r+2 in (Uc, U0c) ---- */
+    auto march_row = [&](int r, const double *Uc, const double *U0c) {
+      xfaces();
+      __builtin_amdgcn_sched_barrier(0);
+
+      /* row r+2: projection */
+      double P2[NV];
+      const int rr = r + 2;
+      {
+        double I2[NV];
+        project<STAGE, true>(Uc, U0c, col_ring3 || rr == ly - 1, dt, dtkn, force, P2, I2, cp);
+#pragma unroll
+        for (int n = 0; n < NV; n++) INIT(rr, n) = I2[n];
+      }
+      if (STAGE == 1 && col_prod && rr <= ly - 2) fold_max(mxp, cp);
+      __builtin_amdgcn_sched_barrier(0);
+
+      /* y-face r+1/2 */
+      {
+        double Tu[NV], Bn[NV], G[NV], s;
+#pragma unroll
+        for (int n = 0; n < NV; n++) Tu[n] = TUP(n);
+#pragma unroll
+        for (int g = 0; g < NV; g += 3) {
+          double e1[3], e2[3], sy[3];
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            e1[k] = Pn[g + k] - Pc[g + k];
+            e2[k] = P2[g + k] - Pn[g + k];
+          }
+          limit3<LIM>(e1, e2, sy);
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            TUP(g + k) = Pn[g + k] + 0.5 * sy[k];
+            Bn[g + k] = Pn[g + k] - 0.5 * sy[k];
+          }
+        }
+        face<1>(Tu, Bn, dtdy, G, s);
+        if (col_face) fold_max(mxy, s);
+        if (st_col && r >= MBC && r <= ly - 1 - MBC) {
+          const unsigned ob = cout + rowb * (unsigned)r;
+#pragma unroll
+          for (int n = 0; n < NV; n++) {
+            const double v = acc[n] - G[n];
+            store1b(rout, n * pb, ob, (STAGE == 2) ? 0.5 * v : v);
+          }
+        }
+#pragma unroll
+        for (int n = 0; n < NV; n++) {
+          acc[n] = INIT(r + 1, n) + G[n];
+          Pc[n] = Pn[n];
+          Pn[n] = P2[n];
+        }
+      }
+    };
+    for (int r = r0; r <= T.r1; r++) {
+      /* row r+2 is loaded while the x-faces of row r are computed */
+      fetch(r + 2, U, U0);
+      march_row(r, U, U0);
+    }
+  } else {
+    /* ---- warm-up: rows r1+2 .. r1-1 (the mirror image of the above) ---- */
+    const int r1 = T.r1;
+    {
+      double Ir[NV];
+      int rr = row_of(r1 + 2);
+      load9b(rin, pb, cin + rowb * rr, U);
+      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
+      rr = row_of(r1 + 1);
+      load9b(rin, pb, cin + rowb * rr, U);
+      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp);
+      rr = row_of(r1);
+      load9b(rin, pb, cin + rowb * rr, U);
+      load9b(rin0, pb, cin + rowb * rr, U0);
+      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp);
+      rr = row_of(r1 - 1);
+      load9b(rin, pb, cin + rowb * rr, U);
+      load9b(rin0, pb, cin + rowb * rr, U0);
+      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp);
+#pragma unroll
+      for (int n = 0; n < NV; n++) INIT(r1 - 1, n) = Ir[n];
+
+      /* face r1+1/2 from the slopes of rows r1+1 and r1, grouped as in the
+       * march, so that it equals the face the tile above computes */
+      double Tl[NV], Bu[NV], G[NV], s;
+#pragma unroll
+      for (int g = 0; g < NV; g += 3) {
+        double e1[3], e2[3], sh[3], sc[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          e1[k] = Pb[g + k] - Pc[g + k];
+          e2[k] = Pa[g + k] - Pb[g + k];
+        }
+        limit3<LIM>(e1, e2, sh);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          e1[k] = Pc[g + k] - Pn[g + k];
+          e2[k] = Pb[g + k] - Pc[g + k];
+        }
+        limit3<LIM>(e1, e2, sc);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          Bu[g + k] = Pb[g + k] - 0.5 * sh[k];
+          Tl[g + k] = Pc[g + k] + 0.5 * sc[k];
+          TUP(g + k) = Pc[g + k] - 0.5 * sc[k];
+        }
+      }
+      face<1>(Tl, Bu, dtdy, G, s);
+      if (col_face) fold_max(mxy, s);
+#pragma unroll
+      for (int n = 0; n < NV; n++) acc[n] -= G[n];
+    }
+
+    /* ---- march down: row r, with the input of row r-2 in (Uc, U0c); a cell
+     * accumulates init - G(j+1/2) + (F(i-1/2) - F(i+1/2)) + G(j-1/2) ---- */
+    auto march_row = [&](int r, const double *Uc, const double *U0c) {
+      xfaces();
+      __builtin_amdgcn_sched_barrier(0);
+
+      /* row r-2 (clamped to ring 3 below r = 1): projection */
+      double P2[NV];
+      {
+        const int rr = row_of(r - 2);
+        double I2[NV];
+        project<STAGE, true>(Uc, U0c, col_ring3 || rr == 0, dt, dtkn, force, P2, I2, cp);
+#pragma unroll
+        for (int n = 0; n < NV; n++) INIT(r - 2, n) = I2[n];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+
+      /* y-face r-1/2 */
+      {
+        double Bu[NV], Tl[NV], G[NV], s;
+#pragma unroll
+        for (int n = 0; n < NV; n++) Bu[n] = TUP(n);
+#pragma unroll
+        for (int g = 0; g < NV; g += 3) {
+          double e1[3], e2[3], sy[3];
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            e1[k] = Pn[g + k] - P2[g + k];
+            e2[k] = Pc[g + k] - Pn[g + k];
+          }
+          limit3<LIM>(e1, e2, sy);
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            TUP(g + k) = Pn[g + k] - 0.5 * sy[k];
+            Tl[g + k] = Pn[g + k] + 0.5 * sy[k];
+          }
+        }
+        face<1>(Tl, Bu, dtdy, G, s);
+        if (col_face && r - 1 >= 1) fold_max(mxy, s);
+        if (st_col && r >= MBC && r <= ly - 1 - MBC) {
+          const unsigned ob = cout + rowb * (unsigned)r;
+#pragma unroll
+          for (int n = 0; n < NV; n++) {
+            const double v = acc[n] + G[n];
+            store1b(rout, n * pb, ob, (STAGE == 2) ? 0.5 * v : v);
+          }
+        }
+#pragma unroll
+        for (int n = 0; n < NV; n++) {
+          acc[n] = INIT(r - 1, n) - G[n];
+          Pc[n] = Pn[n];
+          Pn[n] = P2[n];
+        }
+      }
+    };
+    for (int r = r1; r >= T.r0; r--) {
+      /* row r-2 is loaded while the x-faces of row r are computed */
+      fetch(r - 2, U, U0);
+      march_row(r, U, U0);
+    }
   }
 #undef TUP
 #undef INIT
@@ -717,12 +847,13 @@ template <int STAGE>
 int launch_stage(mbm_ctx *c, const StageArgs &A)
 {
   const dim3 grid((A.ntiles + WAVES_PER_WG - 1) / WAVES_PER_WG), block(64 * WAVES_PER_WG);
+  constexpr bool down = STAGE == 2 && MBM_STAGE2_DOWN;
   switch (c->par.limiter) {
-    case 0: hipLaunchKernelGGL((stage_kernel<STAGE, 0>), grid, block, 0, c->stream, A); break;
-    case 1: hipLaunchKernelGGL((stage_kernel<STAGE, 1>), grid, block, 0, c->stream, A); break;
-    case 2: hipLaunchKernelGGL((stage_kernel<STAGE, 2>), grid, block, 0, c->stream, A); break;
-    case 3: hipLaunchKernelGGL((stage_kernel<STAGE, 3>), grid, block, 0, c->stream, A); break;
-    default: hipLaunchKernelGGL((stage_kernel<STAGE, 4>), grid, block, 0, c->stream, A); break;
+    case 0: hipLaunchKernelGGL((stage_kernel<STAGE, 0, down>), grid, block, 0, c->stream, A); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<STAGE, 1, down>), grid, block, 0, c->stream, A); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<STAGE, 2, down>), grid, block, 0, c->stream, A); break;
+    case 3: hipLaunchKernelGGL((stage_kernel<STAGE, 3, down>), grid, block, 0, c->stream, A); break;
+    default: hipLaunchKernelGGL((stage_kernel<STAGE, 4, down>), grid, block, 0, c->stream, A); break;
   }
   HIPCHK(hipGetLastError());
   return MBM_OK;
