@@ -11,10 +11,14 @@
  * one host thread at a time; work is ordered on the context's HIP stream.
  *
  * Device layout (per layer = DATA, TMP, NEW): one allocation holding every
- * block; block k stores 9 component planes of ly*pitch doubles (ly = ny+6,
- * pitch >= lx = nx+6), cell (a,b) of component m at
- *   base + off[k] + m*plane[k] + b*pitch[k] + a,
- * a,b counted from the outermost ghost ring (0-based). FP64 throughout.
+ * block. Block k stores its ly = ny+6 rows at row positions; a position holds
+ * the 9 components of one row back to back, each row padded to a length
+ * plane[k] >= lx = nx+6. Cell (a,b) of component m is at
+ *   base + off[k] + m*plane[k] + pos_k[b]*pitch[k] + a      (pitch = 9*plane),
+ * a,b counted from the outermost ghost ring (0-based). The row order pos_k
+ * (mbm_layer_rows) keeps adjacent the rows that the stage kernel's tiles
+ * march through at the same time; it follows the tile height, and a change
+ * of tile height moves the layers to the new order. FP64 throughout.
  */
 #ifndef MBMFV_H
 #define MBMFV_H
@@ -95,9 +99,19 @@ int mbm_upload(mbm_ctx *c, int layer, int block, const double *host);
 int mbm_download(mbm_ctx *c, int layer, int block, double *host);
 
 /* Device pointer and geometry of one block of one layer (for callers that
- * keep the state on the device, e.g. a PyTorch wrapper). */
+ * keep the state on the device, e.g. a PyTorch wrapper): component stride
+ * 'plane' and row-position stride 'pitch' in doubles. */
 int mbm_layer_ptr(mbm_ctx *c, int layer, int block, void **dptr,
                   long long *plane, int *pitch);
+
+/* Row order of one block: pos[b] = row position of row b, b < min(n, ny+6).
+ * Valid until the next change of tile height. */
+int mbm_layer_rows(mbm_ctx *c, int block, int *pos, int n);
+
+/* The row order of a block of ny interior rows for tiles of 'band' rows, as
+ * mbm_layer_rows reports it (pos holds ny+6 entries); needs no device.
+ * Returns the number of row positions, or a negative MBM_E_* code. */
+int mbm_row_order(int ny, int band, int *pos);
 
 /* SetBoundary (Boundary.c:173-193): fills the three ghost rings of every
  * block of 'layer' at time t, with the reference's corner semantics. */

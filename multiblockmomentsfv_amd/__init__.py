@@ -103,6 +103,8 @@ def lib():
         "mbm_init": (i, [p]),
         "mbm_upload": (i, [p, i, i, p]),
         "mbm_download": (i, [p, i, i, p]),
+        "mbm_layer_rows": (i, [p, i, p, i]),
+        "mbm_row_order": (i, [i, i, p]),
         "mbm_layer_ptr": (i, [p, i, i, ctypes.POINTER(p), ctypes.POINTER(ctypes.c_longlong),
                               ctypes.POINTER(i)]),
         "mbm_set_boundary": (i, [p, i, d]),
@@ -159,7 +161,8 @@ def exported_symbols():
     """Names the C headers declare (checked against the library by the tests)."""
     return [n for n in ("mbm_last_error", "mbm_create", "mbm_destroy", "mbm_set_stream",
                         "mbm_set_params", "mbm_set_tile_rows", "mbm_tile_rows", "mbm_init", "mbm_upload",
-                        "mbm_download", "mbm_layer_ptr", "mbm_set_boundary",
+                        "mbm_download", "mbm_layer_ptr", "mbm_layer_rows", "mbm_row_order",
+                        "mbm_set_boundary",
                         "mbm_time_update", "mbm_commit", "mbm_steps", "mbm_output_values",
                         "mbm_interior_cells", "mbm_block_dims", "mbm_enable_timing",
                         "mbm_kernel_time", "mbm_layer_elems", "mbm_layer_get", "mbm_layer_set",
@@ -176,6 +179,16 @@ def _check(rc, what):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def row_order(ny, band):
+    """Row positions of a block of ny interior rows in the device layout for
+    tiles of `band` rows (no device needed): (pos per row, number of positions)."""
+    pos = np.empty(ny + 6, dtype=np.int32)
+    n = lib().mbm_row_order(int(ny), int(band), _ptr(pos))
+    if n < 0:
+        _check(n, "mbm_row_order")
+    return pos.astype(np.int64), n
 
 
 # ---------------------------------------------------------------- decks
@@ -411,16 +424,41 @@ class Engine:
         _check(lib().mbm_adj_release(self.h), "mbm_adj_release")
 
     def layer_index(self):
-        """Per block (offset, plane, pitch) of the device layout."""
+        """Per block (offset, plane, pitch, rows) of the device layout: cell
+        (a, b) of component m of block k is element
+        offset + m*plane + rows[b]*pitch + a of a whole layer. The row order
+        follows the tile height (set_tile_rows moves the layers)."""
         out = []
         base = None
-        for k in range(len(self.blocks)):
+        for k, b in enumerate(self.blocks):
             dp, pl, pi = ctypes.c_void_p(), ctypes.c_longlong(), ctypes.c_int()
             _check(lib().mbm_layer_ptr(self.h, 0, k, ctypes.byref(dp), ctypes.byref(pl), ctypes.byref(pi)),
                    "mbm_layer_ptr")
             if base is None:
                 base = dp.value
-            out.append(((dp.value - base) // 8, pl.value, pi.value))
+            rows = np.empty(b["ny"] + 6, dtype=np.int32)
+            _check(lib().mbm_layer_rows(self.h, k, _ptr(rows), len(rows)), "mbm_layer_rows")
+            out.append(((dp.value - base) // 8, pl.value, pi.value, rows.astype(np.int64)))
+        return out
+
+    def layer_to_blocks(self, U):
+        """Whole-layer array (device layout, numpy) -> per block (ny+6, nx+6, 9) arrays."""
+        out = []
+        for (off, plane, pitch, rows), b in zip(self.layer_index(), self.blocks):
+            npos = int(rows.max()) + 1
+            blk = U[off:off + npos * pitch].reshape(npos, NV, plane)
+            out.append(np.ascontiguousarray(blk[rows, :, :b["nx"] + 6].transpose(0, 2, 1)))
+        return out
+
+    def blocks_to_layer(self, blocks, out=None):
+        """Inverse of layer_to_blocks: writes (ny+6, nx+6, 9) arrays into a
+        whole-layer array (a new one, zero between the cells, if out is None)."""
+        if out is None:
+            out = np.zeros(self.layer_elems())
+        for (off, plane, pitch, rows), b, a in zip(self.layer_index(), self.blocks, blocks):
+            npos = int(rows.max()) + 1
+            blk = out[off:off + npos * pitch].reshape(npos, NV, plane)
+            blk[rows, :, :b["nx"] + 6] = np.asarray(a).transpose(0, 2, 1)
         return out
 
     def kernel_time(self):

@@ -2,8 +2,8 @@
 torch.autograd.Function.
 
 The state is a float64 CUDA tensor in the engine's device layout (one whole
-layer, all blocks; ``Engine.layer_index`` gives each block's offset, plane
-stride and row pitch).  ``HeunRun.apply(U0, kn, engine, dts, stride)`` runs
+layer, all blocks; ``Engine.layer_index`` gives each block's offset,
+component stride, row-position stride and row order).  ``HeunRun.apply(U0, kn, engine, dts, stride)`` runs
 the steps of the frozen dt schedule ``dts`` from U0 (its ghost cells are
 recomputed by the ghost fill) and returns the final state; its backward is
 the hand-written adjoint of the C ABI (mbm_adj_backward), giving dJ/dU0 on
@@ -35,14 +35,30 @@ class HeunRun(torch.autograd.Function):
         return gU0, torch.tensor(gkn, dtype=torch.float64, device=gUN.device), None, None, None
 
 
-def block_views(U, engine):
-    """(9, ny+6, nx+6) views of each block inside a layer-layout tensor."""
+def _block_rows(U, engine):
+    """Per block: (rows of the block as (npos, 9, plane) view, row order, nx+6)."""
     out = []
-    for (off, plane, pitch), b in zip(engine.layer_index(), engine.blocks):
-        lx, ly = b["nx"] + 6, b["ny"] + 6
-        v = U[off:off + NV * plane].view(NV, plane)[:, :ly * pitch].view(NV, ly, pitch)[:, :, :lx]
-        out.append(v)
+    for (off, plane, pitch, rows), b in zip(engine.layer_index(), engine.blocks):
+        npos = int(rows.max()) + 1
+        idx = torch.as_tensor(rows, device=U.device)
+        out.append((U[off:off + npos * pitch].view(npos, NV, plane), idx, b["nx"] + 6))
     return out
+
+
+def block_views(U, engine):
+    """(9, ny+6, nx+6) cells of each block of a layer-layout tensor (gathered
+    through the row order; differentiable)."""
+    return [blk.index_select(0, idx)[:, :, :lx].permute(1, 0, 2)
+            for blk, idx, lx in _block_rows(U, engine)]
+
+
+def set_interiors(U, engine, values):
+    """Writes (9, ny, nx) interior values of every block into U in place
+    (differentiable; U must not be a leaf that requires grad)."""
+    for (blk, idx, lx), v in zip(_block_rows(U, engine), values):
+        rows = blk.index_select(0, idx)
+        rows[3:-3, :, 3:lx - 3] = v.permute(1, 0, 2)
+        blk.index_copy_(0, idx, rows)
 
 
 def theta(Ub):
@@ -137,8 +153,8 @@ class TemperatureInversion:
 
     def initial_state(self, theta0):
         U = self.U_init.clone()
-        for v, th, ex in zip(block_views(U, self.engine), theta0, self.extra):
-            v[:, 3:-3, 3:-3] = state_from_theta(th, self.rho, ex)
+        set_interiors(U, self.engine, [state_from_theta(th, self.rho, ex)
+                                       for th, ex in zip(theta0, self.extra)])
         return U
 
     def final_theta(self, theta0, kn):

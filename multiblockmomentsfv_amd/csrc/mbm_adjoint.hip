@@ -41,6 +41,7 @@
 
 #include <algorithm>
 #include <map>
+#include <tuple>
 #include <type_traits>
 
 using namespace mbmi;
@@ -133,21 +134,24 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
   const int a = T.a0 + lane;
   const int ac = a < 0 ? 0 : (a > lx - 1 ? lx - 1 : a);
   const bool own = lane >= 3 && lane <= 60 && a >= 0 && a <= lx - 1;
-  const __amdgpu_buffer_rsrc_t rin = block_rsrc(A.in + B.off, B.plane);
-  const __amdgpu_buffer_rsrc_t rin0 = block_rsrc((STAGE == 2 ? A.in0 : A.in) + B.off, B.plane);
-  const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, B.plane);
-  const __amdgpu_buffer_rsrc_t rgi = block_rsrc(A.gin + B.off, B.plane);
-  const __amdgpu_buffer_rsrc_t rgi0 = block_rsrc((STAGE == 2 ? A.gin0 : A.gin) + B.off, B.plane);
+  const long long nel = (long long)B.npos * NV * B.pitch;
+  const __amdgpu_buffer_rsrc_t rin = block_rsrc(A.in + B.off, nel);
+  const __amdgpu_buffer_rsrc_t rin0 = block_rsrc((STAGE == 2 ? A.in0 : A.in) + B.off, nel);
+  const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, nel);
+  const __amdgpu_buffer_rsrc_t rgi = block_rsrc(A.gin + B.off, nel);
+  const __amdgpu_buffer_rsrc_t rgi0 = block_rsrc((STAGE == 2 ? A.gin0 : A.gin) + B.off, nel);
   const __amdgpu_buffer_rsrc_t rgo = block_rsrc((STAGE == 1 && MERGE ? A.gprev : A.gin) + B.off,
-                                                B.plane);
-  const __amdgpu_buffer_rsrc_t rgy = block_rsrc(A.gpy + B.off, B.plane);
+                                                nel);
+  const __amdgpu_buffer_rsrc_t rgy = block_rsrc(A.gpy + B.off, nel);
   const int pb = (int)(8 * B.plane);
-  const unsigned rowb = 8u * (unsigned)B.pitch, col = 8u * (unsigned)ac;
+  const unsigned col = 8u * (unsigned)ac;
+  /* byte offset of row x (its position in the block's row order) */
+  auto rowb = [&](int x) { return 8u * (unsigned)row_elem(B, x); };
   const double dtdx = A.dt / B.dx, dtkn = A.dt / A.kn;
   double gkn = 0.0;
 
   auto fetch = [&](int r, double *U, double *U0, double *g) {
-    const unsigned boff = col + rowb * (unsigned)r;
+    const unsigned boff = col + rowb(r);
     load9b(rin, pb, boff, U);
     if constexpr (STAGE == 2 && !MERGE) load9b(rin0, pb, boff, U0);   /* DATA part's VJP */
     load9b(rg, pb, boff, g);
@@ -155,7 +159,7 @@ __global__ void __launch_bounds__(64) adj_x(const AdjArgs A)
   /* row r from (U, U0, gr); loads row rn into (Un, U0n, gn) */
   auto row = [&](int r, const double *U, const double *U0, const double *gr, int rn, double *Un,
                  double *U0n, double *gn) {
-    const unsigned boff = col + rowb * (unsigned)r;
+    const unsigned boff = col + rowb(r);
     double gy[NV], o[NV];
     load9b(rgy, pb, boff, gy);
     if constexpr (STAGE == 1) load9b(rgo, pb, boff, o);
@@ -286,23 +290,26 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
   const int ac = a > lx - 1 ? lx - 1 : a;
   const bool colin = act && a >= 1;
   const bool colr3 = a == 0 || a >= lx - 1;
-  const __amdgpu_buffer_rsrc_t rin = block_rsrc(A.in + B.off, B.plane);
-  const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, B.plane);
-  const __amdgpu_buffer_rsrc_t rgy = block_rsrc(A.gpy + B.off, B.plane);
+  const long long nel = (long long)B.npos * NV * B.pitch;
+  const __amdgpu_buffer_rsrc_t rin = block_rsrc(A.in + B.off, nel);
+  const __amdgpu_buffer_rsrc_t rg = block_rsrc(A.gout + B.off, nel);
+  const __amdgpu_buffer_rsrc_t rgy = block_rsrc(A.gpy + B.off, nel);
   const int pb = (int)(8 * B.plane);
-  const unsigned rowb = 8u * (unsigned)B.pitch, col = 8u * (unsigned)ac;
+  const unsigned col = 8u * (unsigned)ac;
+  /* byte offset of row x (its position in the block's row order) */
+  auto rowb = [&](int x) { return 8u * (unsigned)row_elem(B, x); };
   const double dtdy = A.dt / B.dy, dtkn = A.dt / A.kn;
   auto rowc = [&](int r) { return r < 0 ? 0 : (r > ly - 1 ? ly - 1 : r); };
   auto prims = [&](int r, double *P) {
     const int rr = rowc(r);
     double U[NV], dummy[NV], cp;
-    load9b(rin, pb, col + rowb * (unsigned)rr, U);
+    load9b(rin, pb, col + rowb(rr), U);
     project<STAGE, false>(U, U, colr3 || rr == 0 || rr == ly - 1, A.dt, dtkn, A.force, P, dummy,
                           cp);
   };
   auto gweighted = [&](int r, double *g) {
     const int rr = rowc(r);
-    load9b(rg, pb, col + rowb * (unsigned)rr, g);
+    load9b(rg, pb, col + rowb(rr), g);
     const double w = cell_interior(a, r, lx, ly) ? A.scale : 0.0;
 #pragma unroll
     for (int n = 0; n < NV; n++) g[n] *= w;
@@ -354,8 +361,8 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
   /* inputs of iteration it (the state of row k+2, gout of row k+1) are
    * loaded one iteration ahead, into two alternating register sets */
   auto fetch = [&](int it, double *U, double *g) {
-    load9b(rin, pb, col + rowb * (unsigned)rowc(rb + it + 2), U);
-    load9b(rg, pb, col + rowb * (unsigned)rowc(rb + it + 1), g);
+    load9b(rin, pb, col + rowb(rowc(rb + it + 2)), U);
+    load9b(rg, pb, col + rowb(rowc(rb + it + 1)), g);
   };
   auto step = [&](int it, const double *Uc, const double *gc, double *Un, double *gn) {
     const int k = rb + it;
@@ -424,7 +431,7 @@ __global__ void __launch_bounds__(64) adj_y(const AdjArgs A)
     /* row k-1 is complete: its y-part goes to gpy for adj_x */
     const int rf = k - 1;
     if (colin && rf >= rs && rf <= re) {
-      const unsigned boff = col + rowb * (unsigned)rf;
+      const unsigned boff = col + rowb(rf);
 #pragma unroll
       for (int n = 0; n < NV; n++) store1b(rgy, n * pb, boff, gPm[n]);
     }
@@ -572,24 +579,25 @@ int adj_setup(mbm_ctx *c)
   HIPCHK(hipMalloc(&A->knpart, A->nknpart * sizeof(double)));
   /* inverse ghost map: CSR over source cells */
   std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
-  std::map<std::pair<int, int>, std::vector<int>> inv;
+  /* source cells in (block, row, column) order */
+  std::map<std::tuple<int, int, int>, std::vector<int>> inv;
   std::vector<int> dir;
   for (int i = 0; i < (int)gm.size(); i++) {
-    if (gm[i].sblk >= 0) inv[{ gm[i].sblk, gm[i].scell }].push_back(i);
+    if (gm[i].sblk >= 0) inv[{ gm[i].sblk, gm[i].sb, gm[i].sa }].push_back(i);
     else dir.push_back(i);
   }
   auto ghost_at = [&](int i) {
     const GhostDev &q = gm[i];
     const BlockDev &D = c->hblk[q.dblk];
-    return FoldGhost{ D.off + q.dcell, (int)D.plane, q.sign };
+    return FoldGhost{ cell_elem(D, q.db, q.da), (int)D.plane, q.sign };
   };
   std::vector<FoldGhost> fd, fg;
   std::vector<FoldSrc> fs;
   for (int i : dir) fd.push_back(ghost_at(i));
   for (auto &kv : inv) {
-    const BlockDev &S = c->hblk[kv.first.first];
-    FoldSrc src{ S.off + kv.first.second, (int)S.plane, (int)fg.size(), 0, 0,
-                 ghost_at(kv.second[0]) };
+    const BlockDev &S = c->hblk[std::get<0>(kv.first)];
+    FoldSrc src{ cell_elem(S, std::get<1>(kv.first), std::get<2>(kv.first)), (int)S.plane,
+                 (int)fg.size(), 0, 0, ghost_at(kv.second[0]) };
     for (int i : kv.second) fg.push_back(ghost_at(i));
     src.j1 = (int)fg.size();
     fs.push_back(src);

@@ -124,14 +124,14 @@ extern "C" int mbm_batch_download(mbm_ctx *c, int deck, int block, double *host)
   const int par = c->deck_parity.empty() ? 0 : c->deck_parity[deck];
   const int phys = c->role[par ? MBM_L_NEW : MBM_L_DATA];
   const BlockDev &B = c->hblk[k];
-  std::vector<double> soa((size_t)B.plane * NV);
+  std::vector<double> soa((size_t)block_elems(B));
   HIPCHK(hipMemcpyAsync(soa.data(), c->layer[phys] + B.off, soa.size() * sizeof(double),
                         hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int b = 0; b < B.ly; b++)
     for (int a = 0; a < B.lx; a++)
       for (int m = 0; m < NV; m++)
-        host[(size_t)NV * ((size_t)b * B.lx + a) + m] = soa[(size_t)m * B.plane + (size_t)b * B.pitch + a];
+        host[(size_t)NV * ((size_t)b * B.lx + a) + m] = soa[(size_t)(row_elem(B, b) + m * B.plane + a)];
   return MBM_OK;
 }
 

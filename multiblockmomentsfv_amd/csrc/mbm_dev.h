@@ -57,14 +57,14 @@ MBM_DEV void load9(const double *__restrict__ p, long long plane, double *U)
 }
 
 /* Raw buffer access to one block of a layer (gfx9 buffer resource: base,
- * stride 0, byte extent, DATA_FORMAT=32 in the last word). The component
- * offset m*plane*8 goes in the scalar offset and the lane's byte offset in one
- * 32-bit VGPR, so nine loads share one address register; accesses past the
- * extent read 0 / are dropped by the hardware. */
-MBM_DEV __amdgpu_buffer_rsrc_t block_rsrc(const double *base, long long plane)
+ * stride 0, byte extent of `elems` doubles, DATA_FORMAT=32 in the last word).
+ * The component offset m*plane*8 goes in the scalar offset and the lane's
+ * byte offset in one 32-bit VGPR, so nine loads share one address register;
+ * accesses past the extent read 0 / are dropped by the hardware. */
+MBM_DEV __amdgpu_buffer_rsrc_t block_rsrc(const double *base, long long elems)
 {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0,
-                                           (int)(NVC * plane * 8), 0x00020000);
+                                           (int)(elems * 8), 0x00020000);
 }
 
 MBM_DEV void load9b(__amdgpu_buffer_rsrc_t r, int pbytes, unsigned boff, double *U)

@@ -95,13 +95,15 @@ stage_kernel(const StageArgs A)
     in0 = data;
     out = STAGE == 1 ? A.tmp : newb;
   }
-  /* block bases (uniform) and per-lane byte offsets within a component plane;
-   * the host guarantees 8*plane < 2^31 */
-  const __amdgpu_buffer_rsrc_t rin = block_rsrc(in + B.off, B.plane);
-  const __amdgpu_buffer_rsrc_t rin0 = block_rsrc(in0 + B.off, B.plane);
-  const __amdgpu_buffer_rsrc_t rout = block_rsrc(out + B.off, B.plane);
+  /* block bases (uniform) and per-lane byte offsets within a block; the host
+   * guarantees that a block spans less than 2^31 bytes */
+  const long long nel = (long long)B.npos * NV * B.pitch;
+  const __amdgpu_buffer_rsrc_t rin = block_rsrc(in + B.off, nel);
+  const __amdgpu_buffer_rsrc_t rin0 = block_rsrc(in0 + B.off, nel);
+  const __amdgpu_buffer_rsrc_t rout = block_rsrc(out + B.off, nel);
   const int pb = (int)(8 * B.plane);
-  const unsigned rowb = 8u * (unsigned)B.pitch;
+  /* byte offset of row x (its position in the block's row order) */
+  auto rowb = [&](int x) { return 8u * (unsigned)row_elem(B, x); };
   const unsigned cin = 8u * (unsigned)ac, cout = 8u * (unsigned)a;
   const double dtdx = dt / B.dx, dtdy = dt / B.dy;
 
@@ -154,7 +156,7 @@ stage_kernel(const StageArgs A)
   };
   /* input rows, clamped to the block (row 0 and ly-1 are ghost ring 3) */
   auto fetch = [&](int row, double *Ud, double *U0d) {
-    const unsigned roff = cin + rowb * (unsigned)row_of(row);
+    const unsigned roff = cin + rowb(row_of(row));
     load9b(rin, pb, roff, Ud);
     if constexpr (STAGE == 2) load9b(rin0, pb, roff, U0d);
   };
@@ -165,19 +167,19 @@ stage_kernel(const StageArgs A)
     {
       double Ir[NV];
       int rr = row_of(r0 - 2);
-      load9b(rin, pb, cin + rowb * rr, U);
+      load9b(rin, pb, cin + rowb(rr), U);
       project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
       rr = row_of(r0 - 1);
-      load9b(rin, pb, cin + rowb * rr, U);
+      load9b(rin, pb, cin + rowb(rr), U);
       project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp);
       rr = row_of(r0);
-      load9b(rin, pb, cin + rowb * rr, U);
-      if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb * rr, U0);
+      load9b(rin, pb, cin + rowb(rr), U);
+      if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb(rr), U0);
       project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp);
       if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
       rr = row_of(r0 + 1);
-      load9b(rin, pb, cin + rowb * rr, U);
-      if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb * rr, U0);
+      load9b(rin, pb, cin + rowb(rr), U);
+      if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb(rr), U0);
       project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp);
       if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
 #pragma unroll
@@ -253,7 +255,7 @@ stage_kernel(const StageArgs A)
         face<1>(Tu, Bn, dtdy, G, s);
         if (col_face) fold_max(mxy, s);
         if (st_col && r >= MBC && r <= ly - 1 - MBC) {
-          const unsigned ob = cout + rowb * (unsigned)r;
+          const unsigned ob = cout + rowb(r);
 #pragma unroll
           for (int n = 0; n < NV; n++) {
             const double v = acc[n] - G[n];
@@ -279,18 +281,18 @@ stage_kernel(const StageArgs A)
     {
       double Ir[NV];
       int rr = row_of(r1 + 2);
-      load9b(rin, pb, cin + rowb * rr, U);
+      load9b(rin, pb, cin + rowb(rr), U);
       project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
       rr = row_of(r1 + 1);
-      load9b(rin, pb, cin + rowb * rr, U);
+      load9b(rin, pb, cin + rowb(rr), U);
       project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp);
       rr = row_of(r1);
-      load9b(rin, pb, cin + rowb * rr, U);
-      load9b(rin0, pb, cin + rowb * rr, U0);
+      load9b(rin, pb, cin + rowb(rr), U);
+      load9b(rin0, pb, cin + rowb(rr), U0);
       project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp);
       rr = row_of(r1 - 1);
-      load9b(rin, pb, cin + rowb * rr, U);
-      load9b(rin0, pb, cin + rowb * rr, U0);
+      load9b(rin, pb, cin + rowb(rr), U);
+      load9b(rin0, pb, cin + rowb(rr), U0);
       project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp);
 #pragma unroll
       for (int n = 0; n < NV; n++) INIT(r1 - 1, n) = Ir[n];
@@ -366,7 +368,7 @@ stage_kernel(const StageArgs A)
         face<1>(Tl, Bu, dtdy, G, s);
         if (col_face && r - 1 >= 1) fold_max(mxy, s);
         if (st_col && r >= MBC && r <= ly - 1 - MBC) {
-          const unsigned ob = cout + rowb * (unsigned)r;
+          const unsigned ob = cout + rowb(r);
 #pragma unroll
           for (int n = 0; n < NV; n++) {
             const double v = acc[n] + G[n];
@@ -516,6 +518,16 @@ __global__ void zero_kernel(unsigned long long *p, int n)
   if (i < n) p[i] = 0ull;
 }
 
+/* copies every cell of one block between two row orders (n = ly*NV*lx) */
+__global__ void relayout_kernel(double *dst, const double *src, const BlockDev to, const BlockDev from,
+                                int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = i % from.lx, t = i / from.lx, m = t % NV, b = t / NV;
+  dst[cell_elem(to, b, a) + m * to.plane] = src[cell_elem(from, b, a) + m * from.plane];
+}
+
 /* interior initial state (Main.c:126-136) */
 __global__ void init_kernel(double *layer, const BlockDev *blocks, int blk, int nx, int ny,
                             int scenario, const double *tj)
@@ -530,7 +542,7 @@ __global__ void init_kernel(double *layer, const BlockDev *blocks, int blk, int 
   const double x = __dadd_rn(B.bx, __dmul_rn((double)a, B.dx));
   const double y = __dadd_rn(B.by, __dmul_rn((double)b, B.dy));
   mbm_scn_eval(scenario, tjl, x, y, U);
-  double *d = layer + B.off + (long long)b * B.pitch + a;
+  double *d = layer + cell_elem(B, b, a);
   for (int m = 0; m < NV; m++) d[m * B.plane] = U[m];
 }
 
@@ -545,7 +557,7 @@ __global__ void output_kernel(const double *layer, const BlockDev *blocks, int b
   if (p >= (nx + 2) * ni) return;
   const BlockDev B = blocks[blk];
   const int a = MBC - 1 + p / ni, b = MBC - 1 + p % ni;
-  const double *s = layer + B.off + (long long)b * B.pitch + a;
+  const double *s = layer + cell_elem(B, b, a);
   double U[NV];
   for (int m = 0; m < NV; m++) U[m] = s[m * B.plane];
   /* Primitive with true divisions (Flux.c:30-56), output only */
@@ -713,15 +725,16 @@ std::vector<GhostDev> build_ghost_map(const std::vector<mbm_block_desc> &D,
         const Src s = resolve(D, k, a, b, 0);
         GhostDev g{};
         g.dblk = k;
-        g.dcell = b * hb[k].pitch + a;
+        g.da = a;
+        g.db = b;
         g.sign = s.sign;
         if (s.blk >= 0) {
           g.sblk = s.blk;
-          g.scell = s.b * hb[s.blk].pitch + s.a;
+          g.sa = s.a;
+          g.sb = s.b;
         } else {
           const int kk = -s.blk - 1;
           g.sblk = -1;
-          g.scell = 0;
           g.x = hb[kk].bx + s.a * hb[kk].dx;
           g.y = hb[kk].by + s.b * hb[kk].dy;
         }
@@ -814,10 +827,18 @@ int auto_tile_rows(const std::vector<mbm_block_desc> &D, int simds)
   return best;
 }
 
+int relayout(mbm_ctx *c, int band);
+
 int upload_tiles(mbm_ctx *c)
 {
   const int rows = c->tile_rows > 0 ? c->tile_rows
                                     : auto_tile_rows(c->desc, c->cus * 4);
+  /* the layers follow the tile height, so that the rows marched together
+   * stay adjacent */
+  if (std::max(rows, 2) != c->hblk[0].band) {
+    const int rc = relayout(c, rows);
+    if (rc) return rc;
+  }
   c->rows_used = rows;
   std::vector<TileDev> t;
   if (c->ngroups == 2) {
@@ -900,6 +921,85 @@ int upload_params(mbm_ctx *c)
 
 namespace mbmi {
 
+/* Row order and offsets of the blocks of a layer for tiles of height band
+ * (blocks 256-byte aligned); off returns the layer's size in elements. */
+int lay_out(std::vector<BlockDev> &hb, int band, size_t &off)
+{
+  band = std::max(band, 2);
+  off = 0;
+  for (size_t k = 0; k < hb.size(); k++) {
+    BlockDev &b = hb[k];
+    if ((long long)(b.ly - 1) * band >= (1ll << 32))
+      return set_err(MBM_E_ARG, "block %zu: %d rows too many for row bands of %d", k + 1, b.ly, band);
+    set_block_layout(b, band);
+    b.off = (long long)off;
+    /* the kernels address a block with 32-bit byte offsets */
+    if (8 * block_elems(b) + 256 >= (1ll << 31))
+      return set_err(MBM_E_ARG, "block %zu too large (%dx%d)", k + 1, b.lx - 2 * MBC, b.ly - 2 * MBC);
+    off += (size_t)((block_elems(b) + 31) & ~31ll);
+  }
+  return MBM_OK;
+}
+
+/* Ghost fill records of the context's blocks, in block order: element offsets
+ * of component 0 of every ghost and its source. */
+void fill_records(const mbm_ctx *c, std::vector<FillRec> &fr, std::vector<double2> &fxy)
+{
+  const std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
+  fr.resize(gm.size());
+  fxy.resize(gm.size());
+  for (size_t i = 0; i < gm.size(); i++) {
+    const GhostDev &g = gm[i];
+    const BlockDev &D = c->hblk[g.dblk];
+    fr[i].dst = cell_elem(D, g.db, g.da);
+    fr[i].dplane = (int)D.plane;
+    fr[i].src = g.sblk >= 0 ? cell_elem(c->hblk[g.sblk], g.sb, g.sa) : -1;
+    fr[i].splane = g.sblk >= 0 ? (int)c->hblk[g.sblk].plane : 0;
+    fr[i].deck = D.deck;
+    fr[i].sign = g.sign;
+    fxy[i] = make_double2(g.x, g.y);
+  }
+}
+
+/* Moves the three layers to the row order of tile height band; the ghost
+ * records follow, captured graphs and adjoint buffers are dropped. */
+int relayout(mbm_ctx *c, int band)
+{
+  std::vector<BlockDev> nh = c->hblk;
+  size_t off = 0;
+  int rc = lay_out(nh, band, off);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->stream2) HIPCHK(hipStreamSynchronize(c->stream2));
+  if (c->adj && (rc = mbm_adj_release(c))) return rc;
+  release_graphs(c);
+  for (int l = 0; l < 3; l++) {
+    double *nl = nullptr;
+    if (hipMalloc(&nl, off * sizeof(double)) != hipSuccess)
+      return set_err(MBM_E_NOMEM, "device allocation of %zu bytes failed", off * sizeof(double));
+    HIPCHK(hipMemsetAsync(nl, 0, off * sizeof(double), c->stream));
+    for (int k = 0; k < c->nb; k++) {
+      const BlockDev &B = c->hblk[k];
+      const int n = B.ly * NV * B.lx;
+      hipLaunchKernelGGL(relayout_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, nl,
+                         c->layer[l], nh[k], B, n);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(c->layer[l]));
+    c->layer[l] = nl;
+  }
+  c->hblk = nh;
+  c->layer_elems = off;
+  HIPCHK(hipMemcpy(c->dblk, c->hblk.data(), c->nb * sizeof(BlockDev), hipMemcpyHostToDevice));
+  std::vector<FillRec> fr;
+  std::vector<double2> fxy;
+  fill_records(c, fr, fxy);
+  if (!fr.empty())
+    HIPCHK(hipMemcpy(c->dfill, fr.data(), fr.size() * sizeof(FillRec), hipMemcpyHostToDevice));
+  return MBM_OK;
+}
+
 /* Device state of one or more decks whose blocks are listed back to back;
  * deck d owns blocks [deck_first[d], deck_first[d+1]) and its sync codes are
  * already global (1-based over all blocks). */
@@ -934,21 +1034,18 @@ mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_f
   c->deck_first.assign(deck_first, deck_first + ndecks + 1);
   c->deck_par.assign(deck_par, deck_par + ndecks);
   if (!hipok(hipGetDevice(&c->device), "hipGetDevice")) return fail(rc);
-  size_t off = 0;
   for (int d = 0; d < ndecks; d++)
     for (int k = deck_first[d]; k < deck_first[d + 1]; k++) {
       BlockDev b{};
       b.lx = blocks[k].nx + 2 * MBC;
       b.ly = blocks[k].ny + 2 * MBC;
-      b.pitch = (b.lx + 7) & ~7;
-      b.plane = (long long)b.ly * b.pitch;
-      b.plane = (b.plane + 31) & ~31ll;
-      b.off = (long long)off;
       b.deck = d;
       geometry(blocks[k], b.bx, b.by, b.dx, b.dy);
-      off += (size_t)b.plane * NV;
       c->hblk.push_back(b);
     }
+  /* rows ordered for the automatic tile height */
+  size_t off = 0;
+  if ((rc = lay_out(c->hblk, auto_tile_rows(c->desc, c->cus * 4), off))) return fail(rc);
   c->layer_elems = off;
   if (stream) {
     c->stream = (hipStream_t)stream;
@@ -966,21 +1063,10 @@ mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_f
       return fail(rc);
     c->role[l] = l;
   }
-  std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
-  c->nghost = (int)gm.size();
-  std::vector<FillRec> fr(gm.size());
-  std::vector<double2> fxy(gm.size());
-  for (size_t i = 0; i < gm.size(); i++) {
-    const GhostDev &g = gm[i];
-    const BlockDev &D = c->hblk[g.dblk];
-    fr[i].dst = D.off + g.dcell;
-    fr[i].dplane = (int)D.plane;
-    fr[i].src = g.sblk >= 0 ? c->hblk[g.sblk].off + g.scell : -1;
-    fr[i].splane = g.sblk >= 0 ? (int)c->hblk[g.sblk].plane : 0;
-    fr[i].deck = D.deck;
-    fr[i].sign = g.sign;
-    fxy[i] = make_double2(g.x, g.y);
-  }
+  std::vector<FillRec> fr;
+  std::vector<double2> fxy;
+  fill_records(c, fr, fxy);
+  c->nghost = (int)fr.size();
   const bool ok =
       hipok(hipMalloc(&c->dblk, nblocks * sizeof(BlockDev)), "hipMalloc") &&
       hipok(hipMemcpy(c->dblk, c->hblk.data(), nblocks * sizeof(BlockDev), hipMemcpyHostToDevice), "hipMemcpy") &&
@@ -1298,11 +1384,11 @@ extern "C" int mbm_upload(mbm_ctx *c, int layer, int block, const double *host)
   int rc = check_lb(c, layer, block);
   if (rc) return rc;
   const BlockDev &B = c->hblk[block];
-  std::vector<double> soa((size_t)B.plane * NV, 0.0);
+  std::vector<double> soa((size_t)block_elems(B), 0.0);
   for (int b = 0; b < B.ly; b++)
     for (int a = 0; a < B.lx; a++)
       for (int m = 0; m < NV; m++)
-        soa[(size_t)m * B.plane + (size_t)b * B.pitch + a] = host[(size_t)NV * ((size_t)b * B.lx + a) + m];
+        soa[(size_t)(row_elem(B, b) + m * B.plane + a)] = host[(size_t)NV * ((size_t)b * B.lx + a) + m];
   HIPCHK(hipMemcpyAsync(c->layer[c->role[layer]] + B.off, soa.data(), soa.size() * sizeof(double),
                         hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1314,14 +1400,14 @@ extern "C" int mbm_download(mbm_ctx *c, int layer, int block, double *host)
   int rc = check_lb(c, layer, block);
   if (rc) return rc;
   const BlockDev &B = c->hblk[block];
-  std::vector<double> soa((size_t)B.plane * NV);
+  std::vector<double> soa((size_t)block_elems(B));
   HIPCHK(hipMemcpyAsync(soa.data(), c->layer[c->role[layer]] + B.off, soa.size() * sizeof(double),
                         hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int b = 0; b < B.ly; b++)
     for (int a = 0; a < B.lx; a++)
       for (int m = 0; m < NV; m++)
-        host[(size_t)NV * ((size_t)b * B.lx + a) + m] = soa[(size_t)m * B.plane + (size_t)b * B.pitch + a];
+        host[(size_t)NV * ((size_t)b * B.lx + a) + m] = soa[(size_t)(row_elem(B, b) + m * B.plane + a)];
   return MBM_OK;
 }
 
@@ -1333,8 +1419,29 @@ extern "C" int mbm_layer_ptr(mbm_ctx *c, int layer, int block, void **dptr, long
   const BlockDev &B = c->hblk[block];
   if (dptr) *dptr = c->layer[c->role[layer]] + B.off;
   if (plane) *plane = B.plane;
-  if (pitch) *pitch = B.pitch;
+  if (pitch) *pitch = NV * B.pitch;
   return MBM_OK;
+}
+
+extern "C" int mbm_layer_rows(mbm_ctx *c, int block, int *pos, int n)
+{
+  int rc = check_lb(c, MBM_L_DATA, block);
+  if (rc) return rc;
+  if (!pos || n < 0) return set_err(MBM_E_ARG, "invalid row buffer");
+  const BlockDev &B = c->hblk[block];
+  for (int b = 0; b < n && b < B.ly; b++) pos[b] = row_pos(B, b);
+  return MBM_OK;
+}
+
+extern "C" int mbm_row_order(int ny, int band, int *pos)
+{
+  if (ny < 1 || band < 0 || band > 4096 || !pos) return set_err(MBM_E_ARG, "invalid argument");
+  BlockDev B{};
+  B.lx = 1;
+  B.ly = ny + 2 * MBC;
+  set_block_layout(B, std::max(band, 2));
+  for (int b = 0; b < B.ly; b++) pos[b] = row_pos(B, b);
+  return B.npos;
 }
 
 extern "C" int mbm_set_boundary(mbm_ctx *c, int layer, double t)
@@ -1550,7 +1657,6 @@ extern "C" int mbm_host_set_boundary(const mbm_block_desc *blocks, int nblocks,
     BlockDev b{};
     b.lx = blocks[k].nx + 2 * MBC;
     b.ly = blocks[k].ny + 2 * MBC;
-    b.pitch = b.lx;
     geometry(blocks[k], b.bx, b.by, b.dx, b.dy);
     hb.push_back(b);
   }
@@ -1560,13 +1666,16 @@ extern "C" int mbm_host_set_boundary(const mbm_block_desc *blocks, int nblocks,
     const GhostDev &g = gm[i];
     double v[NV];
     if (g.sblk >= 0) {
-      for (int m = 0; m < NV; m++) v[m] = layers[g.sblk][(size_t)NV * g.scell + m];
+      const size_t sc = (size_t)g.sb * hb[g.sblk].lx + g.sa;
+      for (int m = 0; m < NV; m++) v[m] = layers[g.sblk][NV * sc + m];
     } else {
       mbm_scn_eval(p->scenario, p->scenario_params, g.x, g.y, v);
     }
     for (int m = 0; m < NV; m++) vals[i * NV + m] = ((g.sign >> m) & 1) ? -v[m] : v[m];
   }
-  for (size_t i = 0; i < gm.size(); i++)
-    for (int m = 0; m < NV; m++) layers[gm[i].dblk][(size_t)NV * gm[i].dcell + m] = vals[i * NV + m];
+  for (size_t i = 0; i < gm.size(); i++) {
+    const size_t dc = (size_t)gm[i].db * hb[gm[i].dblk].lx + gm[i].da;
+    for (int m = 0; m < NV; m++) layers[gm[i].dblk][NV * dc + m] = vals[i * NV + m];
+  }
   return MBM_OK;
 }

@@ -32,12 +32,52 @@ constexpr int MBC = 3;
 constexpr int STRIP = 60;          /* owned columns per wavefront */
 constexpr int WAVES_PER_WG = 4;
 
+/* A block in a layer buffer. Rows are stored at row positions: a position
+ * holds the 9 components of one row back to back (component m at m*pitch
+ * elements, pitch >= lx), and the positions are ordered so that the rows
+ * which tiles of height `band` march through at the same time are adjacent:
+ * row b >= 1 lies at position ((b-1) % band) * nbands + (b-1) / band, row 0
+ * at band * nbands. */
 struct BlockDev {
   long long off;      /* element offset of block in a layer buffer */
-  long long plane;    /* elements between components */
+  long long plane;    /* elements between components of a cell (= pitch) */
   int lx, ly, pitch, deck;   /* deck: index of the deck the block belongs to */
   double dx, dy, bx, by;
+  int band, nbands;   /* row order, see above */
+  unsigned bmul;      /* ceil(2^32 / band): (y * bmul) >> 32 == y / band */
+  int npos;           /* row positions: band * nbands + 1 */
 };
+
+__host__ __device__ inline int row_pos(const BlockDev &B, int b)
+{
+  if (b <= 0) return B.band * B.nbands;
+  const unsigned y = (unsigned)(b - 1);
+  const unsigned q = (unsigned)(((unsigned long long)y * B.bmul) >> 32);
+  return (int)(y - q * (unsigned)B.band) * B.nbands + (int)q;
+}
+
+/* element offset of row b within the block, and of cell (a, b), component 0,
+ * within the layer */
+__host__ __device__ inline long long row_elem(const BlockDev &B, int b)
+{
+  return (long long)row_pos(B, b) * (NV * (long long)B.pitch);
+}
+__host__ __device__ inline long long cell_elem(const BlockDev &B, int b, int a)
+{
+  return B.off + row_elem(B, b) + a;
+}
+inline long long block_elems(const BlockDev &B) { return (long long)B.npos * NV * B.pitch; }
+
+/* sets pitch, plane and the row order of a block with lx, ly set */
+inline void set_block_layout(BlockDev &B, int band)
+{
+  B.pitch = (B.lx + 7) & ~7;
+  B.plane = B.pitch;
+  B.band = band;
+  B.nbands = (B.ly - 1 + band - 1) / band;
+  B.bmul = (unsigned)((0x100000000ull + (unsigned long long)band - 1) / (unsigned long long)band);
+  B.npos = band * B.nbands + 1;
+}
 
 /* Device-resident state of the Steps controller of one deck (Main.c:189-244). */
 struct DeckState {
@@ -63,8 +103,8 @@ struct TileDev {
 /* one ghost cell: value = sign * source, source is a cell of the same layer
  * or the scenario state at (x,y) */
 struct GhostDev {
-  int dblk, dcell;    /* destination block, b*pitch+a */
-  int sblk, scell;    /* source block/cell, sblk < 0: scenario */
+  int dblk, da, db;   /* destination block, column, row */
+  int sblk, sa, sb;   /* source block, column, row; sblk < 0: scenario */
   int sign;           /* bit m set: component m negated */
   int pad;
   double x, y;

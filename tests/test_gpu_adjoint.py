@@ -28,7 +28,8 @@ def generic_state(mbm, eng, deck, seed=0, amp=1.0):
     from multiblockmomentsfv_amd.adjoint import smooth_fields
     rng = np.random.default_rng(seed)
     U = eng.layer_get(0)
-    for (off, plane, pitch), b, k in zip(eng.layer_index(), deck.blocks, range(deck.nblocks)):
+    cells = eng.layer_to_blocks(U)
+    for b, k in zip(deck.blocks, range(deck.nblocks)):
         bx, by, dx, dy = deck.geom(k)
         X, Y = np.meshgrid(bx + dx * np.arange(3, b["nx"] + 3), by + dy * np.arange(3, b["ny"] + 3))
         dr, v1, v2, d11, d22, t12, q1, q2 = smooth_fields(X, Y, rng, amp)
@@ -40,23 +41,21 @@ def generic_state(mbm, eng, deck, seed=0, amp=1.0):
         E = r * (0.5 * (v1 ** 2 + v2 ** 2) + 1.5 * th)
         comps = [r, r * v1, r * v2, E, r * (t11 + v1 * v1), r * (t12 + v1 * v2), r * (t22 + v2 * v2),
                  q1 + E * v1 + r * (t11 * v1 + t12 * v2), q2 + E * v2 + r * (t12 * v1 + t22 * v2)]
-        ly, lx = b["ny"] + 6, b["nx"] + 6
         for m, c in enumerate(comps):
-            plane_view = U[off + m * plane: off + m * plane + ly * pitch].reshape(ly, pitch)
-            plane_view[3:-3, 3:lx - 3] = c
+            cells[k][3:-3, 3:-3, m] = c
+    U = eng.blocks_to_layer(cells, U)
     eng.layer_set(0, U)
     eng.set_boundary(0, 0.0)
     return eng.layer_get(0)
 
 
 def interior_mask(eng):
-    m = np.zeros(eng.layer_elems())
-    for (off, plane, pitch), b in zip(eng.layer_index(), eng.blocks):
-        ly, lx = b["ny"] + 6, b["nx"] + 6
-        for c in range(9):
-            v = m[off + c * plane: off + c * plane + ly * pitch].reshape(ly, pitch)
-            v[3:-3, 3:lx - 3] = 1.0
-    return m
+    cells = []
+    for b in eng.blocks:
+        a = np.zeros((b["ny"] + 6, b["nx"] + 6, 9))
+        a[3:-3, 3:-3] = 1.0
+        cells.append(a)
+    return eng.blocks_to_layer(cells)
 
 
 def run(eng, U, dts, kn=None, stride=None):
@@ -69,13 +68,7 @@ def run(eng, U, dts, kn=None, stride=None):
 
 
 def gpu_to_blocks(eng, U):
-    out = []
-    for (off, plane, pitch), b in zip(eng.layer_index(), eng.blocks):
-        ly, lx = b["ny"] + 6, b["nx"] + 6
-        out.append(np.ascontiguousarray(np.stack(
-            [U[off + c * plane: off + c * plane + ly * pitch].reshape(ly, pitch)[:, :lx]
-             for c in range(9)], axis=-1)))
-    return out
+    return eng.layer_to_blocks(U)
 
 
 @pytest.mark.parametrize("name", ["box_small", "lshape_small", "periodic", "mixed_bc"])
@@ -169,11 +162,8 @@ def test_zero_step_adjoint_is_fill_transpose(mbm):
         zero = np.zeros_like(u)
         f_u = run(eng, u, [])
         f_0 = run(eng, zero, [])            # Dirichlet ghosts are constants: use differences
-        g = rng.normal(size=mask.size)
-        for (off, plane, pitch), b in zip(eng.layer_index(), eng.blocks):
-            ly, lx = b["ny"] + 6, b["nx"] + 6
-            for c in range(9):
-                g[off + c * plane: off + c * plane + ly * pitch].reshape(ly, pitch)[:, lx:] = 0
+        # a cotangent on every cell (ghosts included), zero between the cells
+        g = eng.blocks_to_layer(eng.layer_to_blocks(rng.normal(size=mask.size)))
         run(eng, u, [])
         gT, _ = eng.adj_backward(g)
         lhs = gT @ u
@@ -219,16 +209,8 @@ def test_one_step_adjoint_vs_oracle_step(mbm, limiter):
     run(eng, U0, [dt])
     gI, _ = eng.adj_backward(w)
     orc = oracle.Oracle(deck.blocks, kn=deck.kn, limiter=limiter, scenario="tempjump")
-    layout = eng.layer_index()
-
     def to_blocks(U):
-        out = []
-        for (off, plane, pitch), b in zip(layout, deck.blocks):
-            ly, lx = b["ny"] + 6, b["nx"] + 6
-            a = np.stack([U[off + c * plane: off + c * plane + ly * pitch].reshape(ly, pitch)[:, :lx]
-                          for c in range(9)], axis=-1)
-            out.append(np.ascontiguousarray(a))
-        return out
+        return eng.layer_to_blocks(U)
 
     def oracle_step(U):
         orc.set(to_blocks(U), layer=0)
@@ -237,12 +219,7 @@ def test_one_step_adjoint_vs_oracle_step(mbm, limiter):
         return to_blocks_inv(orc.get(2))
 
     def to_blocks_inv(blocks):
-        U = np.zeros(U0.size)
-        for (off, plane, pitch), b, a in zip(layout, deck.blocks, blocks):
-            ly, lx = b["ny"] + 6, b["nx"] + 6
-            for c in range(9):
-                U[off + c * plane: off + c * plane + ly * pitch].reshape(ly, pitch)[:, :lx] = a[:, :, c]
-        return U
+        return eng.blocks_to_layer(blocks)
 
     for trial in range(3):
         v = rng.normal(size=U0.size) * mask * (np.abs(U0) + 1e-2)

@@ -32,6 +32,22 @@ def test_library_exports_every_declared_symbol(mbm):
     assert len(declared_functions()) >= 30
 
 
+@pytest.mark.parametrize("ny,band", [(1024, 37), (64, 20), (5, 2), (100, 1), (7, 4096), (2048, 45)])
+def test_row_order_keeps_marching_rows_adjacent(mbm, ny, band):
+    """The device row order is a bijection of the ny+6 rows onto positions,
+    row 0 last, and the rows that tiles of `band` rows (starting at row 1)
+    visit at the same step are consecutive positions, tile by tile."""
+    pos, npos = mbm.row_order(ny, band)
+    ly = ny + 6
+    b = max(band, 2)
+    nb = -(-(ly - 1) // b)
+    assert len(pos) == ly and len(set(pos.tolist())) == ly
+    assert pos.min() >= 0 and pos.max() < npos == b * nb + 1 and pos[0] == b * nb
+    for t in range(b):
+        rows = [1 + k * b + t for k in range(nb) if 1 + k * b + t < ly]
+        assert [int(pos[r]) for r in rows] == list(range(t * nb, t * nb + len(rows)))
+
+
 @pytest.mark.parametrize("name", ["squareSmall", "square4"])
 def test_deck_geometry_matches_golden_zones(mbm, name):
     deck = mbm.read_deck(deck_path(name))
