@@ -293,7 +293,12 @@ class Engine:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module globals may already be gone; the
+        # process is ending and the driver releases the device memory
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def shape(self, k):
         return (self.blocks[k]["ny"] + 6, self.blocks[k]["nx"] + 6, NV)
@@ -512,7 +517,12 @@ class Batch:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module globals may already be gone; the
+        # process is ending and the driver releases the device memory
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def init(self):
         _check(lib().mbm_init(self.h), "mbm_init")
