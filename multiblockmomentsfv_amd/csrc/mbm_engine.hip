@@ -682,8 +682,9 @@ int validate(const mbm_block_desc *blocks, int nb, const mbm_params *p)
     const mbm_block_desc &B = blocks[k];
     if (B.nx < MBC || B.ny < MBC)
       return set_err(MBM_E_ARG, "block %d: grid %dx%d smaller than 3x3", k + 1, B.nx, B.ny);
-    /* the stage kernel addresses a block with 32-bit byte offsets:
-     * 72 * (padded plane) < 2^31 */
+    /* the kernels address a block with 32-bit byte offsets: 72 bytes per
+     * cell of the padded rows < 2^31 (the row order's padding rows are
+     * checked when the layers are laid out) */
     if ((long long)(B.nx + 6 + 7) * (B.ny + 6) * 8 * NV + 256 >= (1ll << 31))
       return set_err(MBM_E_ARG, "block %d too large (%dx%d)", k + 1, B.nx, B.ny);
     if (!(B.xmax > B.xmin) || !(B.ymax > B.ymin))
