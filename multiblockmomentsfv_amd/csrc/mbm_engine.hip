@@ -972,23 +972,29 @@ int relayout(mbm_ctx *c, int band)
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   if (c->stream2) HIPCHK(hipStreamSynchronize(c->stream2));
+  /* all three new layers first: on failure the context keeps its layout */
+  double *nl[3] = { nullptr, nullptr, nullptr };
+  for (int l = 0; l < 3; l++)
+    if (hipMalloc(&nl[l], off * sizeof(double)) != hipSuccess) {
+      for (int q = 0; q < l; q++) (void)hipFree(nl[q]);
+      return set_err(MBM_E_NOMEM, "device allocation of %zu bytes failed", off * sizeof(double));
+    }
   if (c->adj && (rc = mbm_adj_release(c))) return rc;
   release_graphs(c);
   for (int l = 0; l < 3; l++) {
-    double *nl = nullptr;
-    if (hipMalloc(&nl, off * sizeof(double)) != hipSuccess)
-      return set_err(MBM_E_NOMEM, "device allocation of %zu bytes failed", off * sizeof(double));
-    HIPCHK(hipMemsetAsync(nl, 0, off * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(nl[l], 0, off * sizeof(double), c->stream));
     for (int k = 0; k < c->nb; k++) {
       const BlockDev &B = c->hblk[k];
       const int n = B.ly * NV * B.lx;
-      hipLaunchKernelGGL(relayout_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, nl,
+      hipLaunchKernelGGL(relayout_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, nl[l],
                          c->layer[l], nh[k], B, n);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int l = 0; l < 3; l++) {
     HIPCHK(hipFree(c->layer[l]));
-    c->layer[l] = nl;
+    c->layer[l] = nl[l];
   }
   c->hblk = nh;
   c->layer_elems = off;
@@ -1344,10 +1350,13 @@ extern "C" int mbm_set_tile_rows(mbm_ctx *c, int rows)
 {
   if (!c) return set_err(MBM_E_ARG, "NULL context");
   if (rows < 0 || rows > 4096) return set_err(MBM_E_ARG, "tile rows %d outside [0,4096]", rows);
+  const int keep = c->tile_rows;
   c->tile_rows = rows;
   HIPCHK(hipStreamSynchronize(c->stream));
   release_graphs(c);
-  return upload_tiles(c);
+  const int rc = upload_tiles(c);
+  if (rc) c->tile_rows = keep;   /* the layers are unchanged when the move failed */
+  return rc;
 }
 
 extern "C" int mbm_init(mbm_ctx *c)
