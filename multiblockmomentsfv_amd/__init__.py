@@ -181,6 +181,26 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def layer_to_blocks(U, index, blocks):
+    """Per block (ny+6, nx+6, 9) arrays of a whole-layer array, given the
+    layout index (offset, plane, pitch, rows) of every block."""
+    out = []
+    for (off, plane, pitch, rows), b in zip(index, blocks):
+        npos = int(rows.max()) + 1
+        blk = U[off:off + npos * pitch].reshape(npos, NV, plane)
+        out.append(np.ascontiguousarray(blk[rows, :, :b["nx"] + 6].transpose(0, 2, 1)))
+    return out
+
+
+def blocks_to_layer(arrays, index, blocks, out):
+    """Writes per block (ny+6, nx+6, 9) arrays into the whole-layer array out."""
+    for (off, plane, pitch, rows), b, a in zip(index, blocks, arrays):
+        npos = int(rows.max()) + 1
+        blk = out[off:off + npos * pitch].reshape(npos, NV, plane)
+        blk[rows, :, :b["nx"] + 6] = np.asarray(a).transpose(0, 2, 1)
+    return out
+
+
 def row_order(ny, band):
     """Row positions of a block of ny interior rows in the device layout for
     tiles of `band` rows (no device needed): (pos per row, number of positions)."""
@@ -448,23 +468,14 @@ class Engine:
 
     def layer_to_blocks(self, U):
         """Whole-layer array (device layout, numpy) -> per block (ny+6, nx+6, 9) arrays."""
-        out = []
-        for (off, plane, pitch, rows), b in zip(self.layer_index(), self.blocks):
-            npos = int(rows.max()) + 1
-            blk = U[off:off + npos * pitch].reshape(npos, NV, plane)
-            out.append(np.ascontiguousarray(blk[rows, :, :b["nx"] + 6].transpose(0, 2, 1)))
-        return out
+        return layer_to_blocks(U, self.layer_index(), self.blocks)
 
     def blocks_to_layer(self, blocks, out=None):
         """Inverse of layer_to_blocks: writes (ny+6, nx+6, 9) arrays into a
         whole-layer array (a new one, zero between the cells, if out is None)."""
         if out is None:
             out = np.zeros(self.layer_elems())
-        for (off, plane, pitch, rows), b, a in zip(self.layer_index(), self.blocks, blocks):
-            npos = int(rows.max()) + 1
-            blk = out[off:off + npos * pitch].reshape(npos, NV, plane)
-            blk[rows, :, :b["nx"] + 6] = np.asarray(a).transpose(0, 2, 1)
-        return out
+        return blocks_to_layer(blocks, self.layer_index(), self.blocks, out)
 
     def kernel_time(self):
         sm, fm = ctypes.c_double(), ctypes.c_double()

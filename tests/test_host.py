@@ -48,6 +48,27 @@ def test_row_order_keeps_marching_rows_adjacent(mbm, ny, band):
         assert [int(pos[r]) for r in rows] == list(range(t * nb, t * nb + len(rows)))
 
 
+def test_layer_conversion_round_trip(mbm):
+    """Block arrays -> whole layer -> block arrays is the identity, every cell
+    lands at off + m*plane + rows[b]*pitch + a, and nothing else is written."""
+    blocks = [dict(nx=7, ny=5), dict(nx=12, ny=9), dict(nx=3, ny=4)]
+    index, off = [], 0
+    for b in blocks:
+        plane = (b["nx"] + 6 + 7) // 8 * 8
+        rows, npos = mbm.row_order(b["ny"], 4)
+        index.append((off, plane, 9 * plane, rows))
+        off += npos * 9 * plane + 32
+    rng = np.random.default_rng(0)
+    arrays = [rng.normal(size=(b["ny"] + 6, b["nx"] + 6, 9)) for b in blocks]
+    U = mbm.blocks_to_layer(arrays, index, blocks, np.zeros(off))
+    for (o, plane, pitch, rows), b, a in zip(index, blocks, arrays):
+        for bb, aa, m in ((0, 0, 0), (b["ny"] + 5, b["nx"] + 5, 8), (3, 2, 4)):
+            assert U[o + m * plane + rows[bb] * pitch + aa] == a[bb, aa, m]
+    assert np.count_nonzero(U) == sum(a.size for a in arrays)
+    for a, c in zip(arrays, mbm.layer_to_blocks(U, index, blocks)):
+        assert a.tobytes() == c.tobytes()
+
+
 @pytest.mark.parametrize("name", ["squareSmall", "square4"])
 def test_deck_geometry_matches_golden_zones(mbm, name):
     deck = mbm.read_deck(deck_path(name))
