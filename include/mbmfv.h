@@ -105,12 +105,14 @@ int mbm_layer_ptr(mbm_ctx *c, int layer, int block, void **dptr,
                   long long *plane, int *pitch);
 
 /* Row order of one block: pos[b] = row position of row b, b < min(n, ny+6).
- * Valid until the next change of tile height. */
+ * Valid until the next change of tile height. No reference counterpart: the
+ * reference's layers are host arrays in natural row order (Mesh.c:161-164). */
 int mbm_layer_rows(mbm_ctx *c, int block, int *pos, int n);
 
 /* The row order of a block of ny interior rows for tiles of 'band' rows, as
- * mbm_layer_rows reports it (pos holds ny+6 entries); needs no device.
- * Returns the number of row positions, or a negative MBM_E_* code. */
+ * mbm_layer_rows reports it (pos holds ny+6 entries); needs no device and
+ * has no reference counterpart. Returns the number of row positions, or a
+ * negative MBM_E_* code. */
 int mbm_row_order(int ny, int band, int *pos);
 
 /* SetBoundary (Boundary.c:173-193): fills the three ghost rings of every
