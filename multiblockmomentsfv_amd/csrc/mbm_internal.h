@@ -44,7 +44,7 @@ struct BlockDev {
   int lx, ly, pitch, deck;   /* deck: index of the deck the block belongs to */
   double dx, dy, bx, by;
   int band, nbands;   /* row order, see above */
-  unsigned bmul;      /* ceil(2^32 / band): (y * bmul) >> 32 == y / band */
+  unsigned bmul;      /* ceil(2^32 / band): (y * bmul) >> 32 == y / band while y * band < 2^32 */
   int npos;           /* row positions: band * nbands + 1 */
 };
 
@@ -68,7 +68,8 @@ __host__ __device__ inline long long cell_elem(const BlockDev &B, int b, int a)
 }
 inline long long block_elems(const BlockDev &B) { return (long long)B.npos * NV * B.pitch; }
 
-/* sets pitch, plane and the row order of a block with lx, ly set */
+/* sets pitch, plane and the row order of a block with lx, ly set (band >= 2,
+ * so that bmul fits 32 bits) */
 inline void set_block_layout(BlockDev &B, int band)
 {
   B.pitch = (B.lx + 7) & ~7;
