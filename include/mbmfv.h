@@ -205,7 +205,9 @@ int mbm_batch_run(mbm_ctx *c, const double *tend, const double *cflreg, double c
                   int maxsteps, double *t, int *nsteps, int *nrej);
 /* DATA of one block of one deck, AoS as mbm_download */
 int mbm_batch_download(mbm_ctx *c, int deck, int block, double *host);
-/* Times the candidate tile heights on the batch and keeps the fastest. */
+/* Times the candidate tile heights on the batch and keeps the fastest. Each
+ * candidate moves the layers to its row order (the state is kept); it
+ * writes TMP and NEW, and drops a recorded adjoint run. */
 int mbm_batch_autotune(mbm_ctx *c, const int *rows, int nrows, int reps, int *best,
                        double *best_ms);
 
