@@ -147,6 +147,9 @@ int mbm_output_values(mbm_ctx *c, int block, int float32, void *host);
 /* Number of interior cells (tools.c:87-95) and of cells incl. ghosts. */
 long long mbm_interior_cells(const mbm_ctx *c);
 int mbm_block_dims(const mbm_ctx *c, int block, int *nx, int *ny);
+/* HIP device index the context was created on (no reference counterpart:
+ * lets a caller check that its device buffers live on the same GPU). */
+int mbm_device(const mbm_ctx *c);
 
 /* Kernel timing of mbm_time_update calls made while timing is enabled: HIP
  * events on the context's stream bracket each stage launch and the TMP ghost
@@ -157,10 +160,11 @@ int mbm_kernel_time(mbm_ctx *c, double *stage_ms, int *nstage, double *fill_ms,
                     int *nfill);
 
 /* Whole-layer copies in the device layout above (all blocks, layer_elems
- * doubles); src/dst may be host or device pointers. */
+ * doubles); src/dst may be host or device pointers holding n doubles, and n
+ * must equal mbm_layer_elems (it changes with the tile height). */
 long long mbm_layer_elems(const mbm_ctx *c);
-int mbm_layer_get(mbm_ctx *c, int layer, void *dst);
-int mbm_layer_set(mbm_ctx *c, int layer, const void *src);
+int mbm_layer_get(mbm_ctx *c, int layer, void *dst, long long n);
+int mbm_layer_set(mbm_ctx *c, int layer, const void *src, long long n);
 
 /* ---- discrete adjoint of the Heun step --------------------------------
  * No counterpart in the reference (it has no derivatives). Reverse-mode
@@ -184,8 +188,10 @@ int mbm_adj_schedule(mbm_ctx *c, double *dts, int cap, int *n);
 /* g_final: dJ/d(final DATA), layer layout (ghost entries are folded through
  * the last ghost fill). g_initial receives dJ/d(initial DATA) on interior
  * cells (ghost entries 0: the initial ghosts are a function of the interior
- * through SetBoundary); *g_kn receives dJ/dKn. Host or device pointers. */
-int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial, double *g_kn);
+ * through SetBoundary); *g_kn receives dJ/dKn. Host or device pointers of n
+ * doubles each, n == mbm_layer_elems. */
+int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial, long long n,
+                     double *g_kn);
 int mbm_adj_release(mbm_ctx *c);
 
 /* ---- batches of independent decks ---------------------------------------

@@ -116,15 +116,16 @@ def lib():
         "mbm_output_values": (i, [p, i, i, p]),
         "mbm_interior_cells": (ctypes.c_longlong, [p]),
         "mbm_block_dims": (i, [p, i, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "mbm_device": (i, [p]),
         "mbm_enable_timing": (i, [p, i]),
         "mbm_kernel_time": (i, [p, ctypes.POINTER(d), ctypes.POINTER(i), ctypes.POINTER(d),
                                 ctypes.POINTER(i)]),
         "mbm_layer_elems": (ctypes.c_longlong, [p]),
-        "mbm_layer_get": (i, [p, i, p]),
-        "mbm_layer_set": (i, [p, i, p]),
+        "mbm_layer_get": (i, [p, i, p, ctypes.c_longlong]),
+        "mbm_layer_set": (i, [p, i, p, ctypes.c_longlong]),
         "mbm_adj_forward": (i, [p, d, d, d, d, i, p, i, i, ctypes.POINTER(i), ctypes.POINTER(d)]),
         "mbm_adj_schedule": (i, [p, p, i, ctypes.POINTER(i)]),
-        "mbm_adj_backward": (i, [p, p, p, ctypes.POINTER(d)]),
+        "mbm_adj_backward": (i, [p, p, p, ctypes.c_longlong, ctypes.POINTER(d)]),
         "mbm_adj_release": (i, [p]),
         "mbm_batch_create": (i, [ctypes.POINTER(BlockDesc), p, i, ctypes.POINTER(Params), p,
                                  ctypes.POINTER(p)]),
@@ -164,7 +165,7 @@ def exported_symbols():
                         "mbm_download", "mbm_layer_ptr", "mbm_layer_rows", "mbm_row_order",
                         "mbm_set_boundary",
                         "mbm_time_update", "mbm_commit", "mbm_steps", "mbm_output_values",
-                        "mbm_interior_cells", "mbm_block_dims", "mbm_enable_timing",
+                        "mbm_interior_cells", "mbm_block_dims", "mbm_device", "mbm_enable_timing",
                         "mbm_kernel_time", "mbm_layer_elems", "mbm_layer_get", "mbm_layer_set",
                         "mbm_adj_forward", "mbm_adj_schedule", "mbm_adj_backward",
                         "mbm_adj_release", "mbm_batch_create", "mbm_batch_ndecks",
@@ -300,6 +301,7 @@ class Engine:
                             ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)),
                "mbm_create")
         self.h = h
+        self.adj_runs = 0          # recordings made by adj_forward (one is kept)
 
     @classmethod
     def from_deck(cls, deck, scenario="riemann2d", **kw):
@@ -326,6 +328,7 @@ class Engine:
     def set_tile_rows(self, rows):
         """Rows per wavefront of the stage kernel; 0 = automatic."""
         _check(lib().mbm_set_tile_rows(self.h, int(rows)), "mbm_set_tile_rows")
+        self.adj_runs += 1         # a tile-height change drops the recorded run
 
     def tile_rows(self):
         return lib().mbm_tile_rows(self.h)
@@ -390,18 +393,48 @@ class Engine:
     def layer_elems(self):
         return lib().mbm_layer_elems(self.h)
 
+    def _layer_buffer(self, a, what, writable):
+        """(pointer, keep-alive object) of a whole-layer buffer: a float64
+        numpy array or torch tensor of exactly layer_elems() contiguous
+        elements (a torch tensor on this engine's device)."""
+        n = self.layer_elems()
+        if hasattr(a, "data_ptr"):
+            import torch
+            if a.dtype != torch.float64 or not a.is_contiguous() or a.numel() != n:
+                raise MbmError(f"{what}: need a contiguous float64 tensor of {n} elements, got "
+                               f"{a.dtype} {tuple(a.shape)} contiguous={a.is_contiguous()}")
+            if a.is_cuda and a.device.index != self.device():
+                raise MbmError(f"{what}: tensor on cuda:{a.device.index}, engine on cuda:{self.device()}")
+            return a.data_ptr(), a
+        if writable:
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+                raise MbmError(f"{what}: need a C-contiguous float64 numpy array")
+            arr = a
+        else:
+            arr = np.ascontiguousarray(a, dtype=np.float64)
+        if arr.size != n:
+            raise MbmError(f"{what}: buffer holds {arr.size} doubles, the layout needs {n}")
+        return arr.ctypes.data, arr
+
+    def device(self):
+        return lib().mbm_device(self.h)
+
     def layer_get(self, layer, out=None):
-        """Copies a whole layer (device layout) to `out` (numpy array or a
-        pointer-like object with .data_ptr(), e.g. a torch tensor)."""
+        """Copies a whole layer (device layout) to `out` (a float64 numpy array
+        or torch tensor of layer_elems() elements)."""
         if out is None:
             out = np.empty(self.layer_elems())
-        ptr = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
-        _check(lib().mbm_layer_get(self.h, layer, ctypes.c_void_p(ptr)), "mbm_layer_get")
+        ptr, keep = self._layer_buffer(out, "layer_get", True)
+        _check(lib().mbm_layer_get(self.h, layer, ctypes.c_void_p(ptr), self.layer_elems()),
+               "mbm_layer_get")
+        del keep
         return out
 
     def layer_set(self, layer, src):
-        ptr = src.data_ptr() if hasattr(src, "data_ptr") else np.ascontiguousarray(src).ctypes.data
-        _check(lib().mbm_layer_set(self.h, layer, ctypes.c_void_p(ptr)), "mbm_layer_set")
+        ptr, keep = self._layer_buffer(src, "layer_set", False)
+        _check(lib().mbm_layer_set(self.h, layer, ctypes.c_void_p(ptr), self.layer_elems()),
+               "mbm_layer_set")
+        del keep
 
     def set_kn(self, kn):
         self.params.kn = float(kn)
@@ -415,6 +448,7 @@ class Engine:
         """Advances DATA recording (dts=None) or replaying a dt schedule,
         with checkpoints every `stride` steps. Returns (nsteps, t_end)."""
         n, te = ctypes.c_int(), ctypes.c_double()
+        self.adj_runs += 1
         if dts is not None:
             dts = np.ascontiguousarray(dts, dtype=np.float64)
             stride = stride or max(1, int(np.ceil(np.sqrt(len(dts)))))
@@ -435,18 +469,20 @@ class Engine:
 
     def adj_backward(self, g_final, g_initial=None):
         """dJ/d(initial DATA) and dJ/dKn from dJ/d(final DATA) (layer layout;
-        numpy arrays or torch tensors)."""
+        float64 numpy arrays or torch tensors of layer_elems() elements)."""
         if g_initial is None:
             g_initial = np.empty(self.layer_elems())
-        gp = g_final.data_ptr() if hasattr(g_final, "data_ptr") else np.ascontiguousarray(g_final).ctypes.data
-        ip = g_initial.data_ptr() if hasattr(g_initial, "data_ptr") else g_initial.ctypes.data
+        gp, keep_g = self._layer_buffer(g_final, "adj_backward g_final", False)
+        ip, keep_i = self._layer_buffer(g_initial, "adj_backward g_initial", True)
         gk = ctypes.c_double()
         _check(lib().mbm_adj_backward(self.h, ctypes.c_void_p(gp), ctypes.c_void_p(ip),
-                                      ctypes.byref(gk)), "mbm_adj_backward")
+                                      self.layer_elems(), ctypes.byref(gk)), "mbm_adj_backward")
+        del keep_g, keep_i
         return g_initial, gk.value
 
     def adj_release(self):
         _check(lib().mbm_adj_release(self.h), "mbm_adj_release")
+        self.adj_runs += 1
 
     def layer_index(self):
         """Per block (offset, plane, pitch, rows) of the device layout: cell

@@ -16,22 +16,39 @@ from . import L_DATA, NV, Engine, read_deck
 
 
 class HeunRun(torch.autograd.Function):
+    """The engine holds one recorded run at a time. forward keeps U0, Kn, the
+    schedule and the number of the recording it made; if another run was
+    recorded on the same engine before this graph's backward (a second
+    objective, or a no-grad evaluation), backward records this run again
+    from the kept U0 before reversing it."""
+
     @staticmethod
-    def forward(ctx, U0, kn, engine, dts, stride):
+    def _record(engine, U0, kn, dts, stride):
         engine.set_stream(torch.cuda.current_stream().cuda_stream)
         engine.set_kn(float(kn))
-        engine.layer_set(L_DATA, U0.detach().contiguous())
+        engine.layer_set(L_DATA, U0)
         engine.set_boundary(L_DATA, 0.0)
         engine.adj_forward(tend=float("inf"), cflreg=1.0, dts=dts, stride=stride)
+        return engine.adj_runs
+
+    @staticmethod
+    def forward(ctx, U0, kn, engine, dts, stride):
+        U0c = U0.detach().contiguous()
+        run = HeunRun._record(engine, U0c, kn, dts, stride)
         UN = torch.empty_like(U0)
         engine.layer_get(L_DATA, UN)
-        ctx.engine = engine
+        ctx.engine, ctx.run, ctx.kn, ctx.dts, ctx.stride = engine, run, float(kn), dts, stride
+        ctx.save_for_backward(U0c)
         return UN
 
     @staticmethod
     def backward(ctx, gUN):
+        eng = ctx.engine
+        if eng.adj_runs != ctx.run:
+            (U0c,) = ctx.saved_tensors
+            ctx.run = HeunRun._record(eng, U0c, ctx.kn, ctx.dts, ctx.stride)
         gU0 = torch.empty_like(gUN)
-        _, gkn = ctx.engine.adj_backward(gUN.contiguous(), gU0)
+        _, gkn = eng.adj_backward(gUN.contiguous(), gU0)
         return gU0, torch.tensor(gkn, dtype=torch.float64, device=gUN.device), None, None, None
 
 

@@ -754,18 +754,30 @@ void free_checkpoints(mbm_adj_state *A)
 
 extern "C" long long mbm_layer_elems(const mbm_ctx *c) { return c ? (long long)c->layer_elems : 0; }
 
-extern "C" int mbm_layer_get(mbm_ctx *c, int layer, void *dst)
+namespace {
+int check_layer_count(const mbm_ctx *c, long long n)
+{
+  if (n != (long long)c->layer_elems)
+    return set_err(MBM_E_ARG, "layer buffer holds %lld doubles, the layout needs %zu", n,
+                   c->layer_elems);
+  return MBM_OK;
+}
+} // namespace
+
+extern "C" int mbm_layer_get(mbm_ctx *c, int layer, void *dst, long long n)
 {
   if (!c || !dst || layer < 0 || layer > 2) return set_err(MBM_E_ARG, "invalid argument");
+  if (int rc = check_layer_count(c, n)) return rc;
   HIPCHK(hipMemcpyAsync(dst, c->layer[c->role[layer]], c->layer_elems * sizeof(double),
                         hipMemcpyDefault, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return MBM_OK;
 }
 
-extern "C" int mbm_layer_set(mbm_ctx *c, int layer, const void *src)
+extern "C" int mbm_layer_set(mbm_ctx *c, int layer, const void *src, long long n)
 {
   if (!c || !src || layer < 0 || layer > 2) return set_err(MBM_E_ARG, "invalid argument");
+  if (int rc = check_layer_count(c, n)) return rc;
   HIPCHK(hipMemcpyAsync(c->layer[c->role[layer]], src, c->layer_elems * sizeof(double),
                         hipMemcpyDefault, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -834,9 +846,12 @@ extern "C" int mbm_adj_schedule(mbm_ctx *c, double *dts, int cap, int *n)
   return MBM_OK;
 }
 
-extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial, double *g_kn)
+extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial, long long n,
+                                double *g_kn)
 {
-  if (!c || !c->adj || c->adj->nckpt == 0) return set_err(MBM_E_STATE, "no recorded forward run");
+  if (!c || !g_final || !g_initial) return set_err(MBM_E_ARG, "NULL argument");
+  if (int rc = check_layer_count(c, n)) return rc;
+  if (!c->adj || c->adj->nckpt == 0) return set_err(MBM_E_STATE, "no recorded forward run");
   mbm_adj_state *A = c->adj;
   const size_t bytes = c->layer_elems * sizeof(double);
   const int N = (int)A->dts.size(), S = A->stride;

@@ -102,7 +102,7 @@ extern "C" int mbm_batch_run(mbm_ctx *c, const double *tend, const double *cflre
                         c->stream));
   /* roles stay fixed in a batch: each deck's parity says where its DATA is */
   const int ndecks = c->ndecks;
-  int rc = device_steps(c, 64);
+  int rc = device_steps(c, 64, st.data());
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(st.data(), c->ddecks, st.size() * sizeof(DeckState), hipMemcpyDeviceToHost,
                         c->stream));

@@ -834,14 +834,8 @@ int upload_tiles(mbm_ctx *c)
 {
   const int rows = c->tile_rows > 0 ? c->tile_rows
                                     : auto_tile_rows(c->desc, c->cus * 4);
-  /* the layers follow the tile height, so that the rows marched together
-   * stay adjacent */
-  if (std::max(rows, 2) != c->hblk[0].band) {
-    const int rc = relayout(c, rows);
-    if (rc) return rc;
-  }
-  c->rows_used = rows;
   std::vector<TileDev> t;
+  int grp[3] = { 0, 0, 0 };
   if (c->ngroups == 2) {
     /* tiles come in block order: group 0's blocks first */
     const std::vector<TileDev> all = build_tiles(c->desc, rows);
@@ -849,19 +843,37 @@ int upload_tiles(mbm_ctx *c)
     std::vector<TileDev> t0, t1;
     for (const TileDev &d : all) (d.blk < b1 ? t0 : t1).push_back(d);
     t = xcd_order(t0);
-    c->grp_tile[0] = 0;
-    c->grp_tile[1] = (int)t.size();
+    grp[1] = (int)t.size();
     const std::vector<TileDev> o1 = xcd_order(t1);
     t.insert(t.end(), o1.begin(), o1.end());
-    c->grp_tile[2] = (int)t.size();
+    grp[2] = (int)t.size();
   } else {
     t = xcd_order(build_tiles(c->desc, rows));
   }
-  if (c->dtiles) HIPCHK(hipFree(c->dtiles));
-  c->dtiles = nullptr;
-  HIPCHK(hipMalloc(&c->dtiles, t.size() * sizeof(TileDev)));
-  HIPCHK(hipMemcpy(c->dtiles, t.data(), t.size() * sizeof(TileDev), hipMemcpyHostToDevice));
+  /* the new tile list is on the device before anything of the context
+   * changes: a failure leaves tiles, layout and tile height as they were */
+  TileDev *nt = nullptr;
+  if (hipMalloc(&nt, t.size() * sizeof(TileDev)) != hipSuccess)
+    return set_err(MBM_E_NOMEM, "device allocation of the tile list failed");
+  if (hipMemcpy(nt, t.data(), t.size() * sizeof(TileDev), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(nt);
+    return set_err(MBM_E_HIP, "upload of the tile list failed");
+  }
+  /* the layers follow the tile height, so that the rows marched together
+   * stay adjacent */
+  if (std::max(rows, 2) != c->hblk[0].band) {
+    const int rc = relayout(c, rows);
+    if (rc) {
+      (void)hipFree(nt);
+      return rc;
+    }
+  }
+  if (c->dtiles) (void)hipFree(c->dtiles);
+  c->dtiles = nt;
   c->ntiles = (int)t.size();
+  c->rows_used = rows;
+  if (c->ngroups == 2)
+    for (int g = 0; g < 3; g++) c->grp_tile[g] = grp[g];
   return MBM_OK;
 }
 
@@ -979,21 +991,28 @@ int relayout(mbm_ctx *c, int band)
       for (int q = 0; q < l; q++) (void)hipFree(nl[q]);
       return set_err(MBM_E_NOMEM, "device allocation of %zu bytes failed", off * sizeof(double));
     }
-  if (c->adj && (rc = mbm_adj_release(c))) return rc;
+  auto drop_new = [&](int code) {
+    (void)hipStreamSynchronize(c->stream);
+    for (int q = 0; q < 3; q++) (void)hipFree(nl[q]);
+    return code;
+  };
+  if (c->adj && (rc = mbm_adj_release(c))) return drop_new(rc);
   release_graphs(c);
   for (int l = 0; l < 3; l++) {
-    HIPCHK(hipMemsetAsync(nl[l], 0, off * sizeof(double), c->stream));
-    for (int k = 0; k < c->nb; k++) {
+    hipError_t e = hipMemsetAsync(nl[l], 0, off * sizeof(double), c->stream);
+    for (int k = 0; k < c->nb && e == hipSuccess; k++) {
       const BlockDev &B = c->hblk[k];
       const int n = B.ly * NV * B.lx;
       hipLaunchKernelGGL(relayout_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, nl[l],
                          c->layer[l], nh[k], B, n);
+      e = hipGetLastError();
     }
-    HIPCHK(hipGetLastError());
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess)
+      return drop_new(set_err(MBM_E_HIP, "relayout failed: %s", hipGetErrorString(e)));
   }
-  HIPCHK(hipStreamSynchronize(c->stream));
   for (int l = 0; l < 3; l++) {
-    HIPCHK(hipFree(c->layer[l]));
+    (void)hipFree(c->layer[l]);
     c->layer[l] = nl[l];
   }
   c->hblk = nh;
@@ -1208,28 +1227,43 @@ int capture(mbm_ctx *c, hipStream_t s, hipGraphExec_t *out, F enq)
   e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return set_err(MBM_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+  /* move the executable graph to the device now, not at its first launch */
+  e = hipGraphUpload(*out, s);
+  if (e != hipSuccess) return set_err(MBM_E_HIP, "graph upload failed: %s", hipGetErrorString(e));
   return MBM_OK;
 }
 
-int device_steps(mbm_ctx *c, int maxchunk)
+/* the captured attempt for DATA in buffer iD and NEW in buffer iN */
+int step_graph(mbm_ctx *c, int iD, int iN, int iT)
+{
+  if (c->graph_ok[iD][iN]) return MBM_OK;
+  double *buf0 = c->layer[iD], *buf1 = c->layer[iN], *tmp = c->layer[iT];
+  int rc;
+  if (c->ngroups == 2)   /* group 0: stage 1, stage 2, controller */
+    rc = capture(c, c->stream, &c->graph[iD][iN], [&] {
+      int r = enqueue_group_stage(c, 0, 1, buf0, buf1, tmp);
+      if (!r) r = enqueue_group_stage(c, 0, 2, buf0, buf1, tmp);
+      return r ? r : enqueue_group_control(c, 0);
+    });
+  else
+    rc = capture(c, c->stream, &c->graph[iD][iN],
+                 [&] { return enqueue_device_step(c, buf0, buf1, tmp); });
+  if (rc) return rc;
+  c->graph_ok[iD][iN] = true;
+  return MBM_OK;
+}
+
+int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known)
 {
   const int iD = c->role[MBM_L_DATA], iN = c->role[MBM_L_NEW], iT = c->role[MBM_L_TMP];
   double *buf0 = c->layer[iD], *buf1 = c->layer[iN], *tmp = c->layer[iT];
   const bool two = c->ngroups == 2;
   int rc;
-  if (!c->graph_ok[iD][iN]) {
-    if (two)   /* group 0: stage 1, stage 2, controller */
-      rc = capture(c, c->stream, &c->graph[iD][iN], [&] {
-        int r = enqueue_group_stage(c, 0, 1, buf0, buf1, tmp);
-        if (!r) r = enqueue_group_stage(c, 0, 2, buf0, buf1, tmp);
-        return r ? r : enqueue_group_control(c, 0);
-      });
-    else
-      rc = capture(c, c->stream, &c->graph[iD][iN],
-                   [&] { return enqueue_device_step(c, buf0, buf1, tmp); });
-    if (rc) return rc;
-    c->graph_ok[iD][iN] = true;
-  }
+  if ((rc = step_graph(c, iD, iN, iT))) return rc;
+  /* a plain context swaps the DATA and NEW roles after an odd number of
+   * accepted steps: the graph of the swapped assignment is captured now, so
+   * that no later call captures one between its launches */
+  if (!c->batch && (rc = step_graph(c, iN, iD, iT))) return rc;
   if (two && !c->graph2_ok) {
     /* group 1, half an attempt later: stage 2, controller, next stage 1 */
     rc = capture(c, c->stream2, &c->graph2, [&] {
@@ -1241,8 +1275,14 @@ int device_steps(mbm_ctx *c, int maxchunk)
     c->graph2_ok = true;
   }
   std::vector<DeckState> st(c->ndecks);
-  HIPCHK(hipMemcpyAsync(st.data(), c->ddecks, c->ndecks * sizeof(DeckState), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (known) {
+    /* the caller has just written these states to the device */
+    st.assign(known, known + c->ndecks);
+  } else {
+    HIPCHK(hipMemcpyAsync(st.data(), c->ddecks, c->ndecks * sizeof(DeckState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (two) HIPCHK(hipStreamSynchronize(c->stream));
   if (two) {
     /* group 1's first stage 1 (the stream is idle: the host has synchronised
      * with everything queued on c->stream) */
@@ -1567,7 +1607,7 @@ extern "C" int mbm_steps(mbm_ctx *c, double *t, double tend, double cflmax, doub
   S.done = !(*t < tend);
   S.reccap = rec ? reccap : 0;
   HIPCHK(hipMemcpyAsync(c->ddecks, &S, sizeof S, hipMemcpyHostToDevice, c->stream));
-  int rc = device_steps(c, 64);
+  int rc = device_steps(c, 64, &S);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(&S, c->ddecks, sizeof S, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1629,6 +1669,8 @@ extern "C" int mbm_block_dims(const mbm_ctx *c, int block, int *nx, int *ny)
   if (ny) *ny = c->desc[block].ny;
   return MBM_OK;
 }
+
+extern "C" int mbm_device(const mbm_ctx *c) { return c ? c->device : -1; }
 
 extern "C" int mbm_enable_timing(mbm_ctx *c, int on)
 {

@@ -208,7 +208,8 @@ int enqueue_stage(mbm_ctx *c, int stage, const double *in, const double *in0, do
                   double dt, unsigned long long *slots);
 mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_first, int ndecks,
                     const mbm_params *deck_par, void *stream, int *rc);
-int device_steps(mbm_ctx *c, int maxchunk);
+/* known: the deck states just written to the device (NULL: read them back) */
+int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known);
 int enqueue_time_update(mbm_ctx *c, double dt);   /* both stages and fills, CFL to device */
 int upload_tiles(mbm_ctx *c);
 int validate(const mbm_block_desc *blocks, int nb, const mbm_params *p);

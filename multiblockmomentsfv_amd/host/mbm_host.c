@@ -166,7 +166,7 @@ int mbm_deck_read(const char *path, mbm_deck *d, int refine_override, int verbos
                 &id, &x0, &x1, &y0, &y1, &nx, &ny, &b0, &b1, &b2, &b3) != EOF) {
     if (id < 1 || id > MBM_MAX_BLOCKS) {
       fprintf(stderr, "   ERROR: block number %d out of range [1,%d=NB]. "
-              "Increase NB in defs.h or fix the block file.\n", id, MBM_MAX_BLOCKS);
+              "Fix the block file (this build allows at most %d blocks).\n", id, MBM_MAX_BLOCKS, MBM_MAX_BLOCKS);
       fclose(fp);
       return fail(MBM_E_FORMAT, "block number %d out of range [1,%d]", id, MBM_MAX_BLOCKS);
     }
