@@ -1,109 +1,162 @@
-"""Numeric comparison of two .hdr/.dat solution files.
+"""Regression check of one solution file pair against another.
 
-Same rules and exit codes as the reference's regression gate
-(tools/compare_solution.py:30-122):
-  * each argument may be the .hdr, the .dat or the shared basename;
-  * payload precision from the [BINARY] PRECISION field (Float64 if absent);
-  * zones are matched by their (block, I, J, NVARS) fingerprint, STRANDID,
-    SOLUTIONTIME and the provenance lines are ignored;
-  * a value passes if |new - ref| <= abstol + reltol*|ref|; the defaults are
-    1e-6/1e-5 when either side is Float32, else 1e-10/1e-8;
-  * exit 0 = PASS, 1 = FAIL, 2 = usage.
+Usage (the argument order and exit codes of the reference's `make check`
+gate, tools/compare_solution.py:80-122):
 
-    python -m multiblockmomentsfv_amd.compare_solution <new> <ref> [abstol] [reltol]
+    python -m multiblockmomentsfv_amd.compare_solution NEW REF [ABSTOL [RELTOL]]
+
+NEW and REF each name a solution written by the output writer: the `.hdr`
+manifest, the `.dat` payload, or their common stem.  The contract kept from
+the reference gate:
+
+* the payload precision is the manifest's `PRECISION = ...` entry, Float64
+  when the entry is missing;
+* the zones must have the same structure, zone by zone in file order:
+  (block, I, J, NVARS).  Everything else in the manifest (STRANDID,
+  SOLUTIONTIME, provenance) is free to differ;
+* a value fails when |new - ref| > abstol + reltol * |ref| (so a NaN on
+  either side does not fail, as in the reference gate);
+* default tolerances: 1e-6 / 1e-5 if either payload is Float32, otherwise
+  1e-10 / 1e-8;
+* exit status 0 = every value within tolerance, 1 = a mismatch, 2 = fewer
+  than two paths (arguments after RELTOL are ignored).
+
+Implementation: one pass over each manifest builds a `Solution` (precision
+and zone table); each zone's values are mapped from the payload with
+np.memmap and compared as float64 arrays, zone by zone.
 """
-import re
+import os
 import sys
+from dataclasses import dataclass, field
 
 import numpy as np
 
+STATUS_PASS, STATUS_FAIL, STATUS_USAGE = 0, 1, 2
 
-def companion(path):
-    base = path[:-4] if path.endswith((".hdr", ".dat")) else path
-    return base + ".hdr", base + ".dat"
-
-
-def precision(hdr):
-    with open(hdr) as f:
-        for line in f:
-            m = re.match(r"\s*PRECISION\s*=\s*(\S+)", line)
-            if m:
-                return m.group(1)
-    return "Float64"
+# tolerance defaults keyed by "is either payload single precision"
+DEFAULT_TOL = {True: (1e-6, 1e-5), False: (1e-10, 1e-8)}
 
 
-def zones(hdr):
-    out = []
-    with open(hdr) as f:
-        for line in f:
-            s = line.strip()
-            if not s.startswith("ZONE"):
-                continue
+@dataclass
+class Zone:
+    block: int
+    ni: int
+    nj: int
+    nvars: int
+    offset: int
 
-            def g(k):
-                return int(re.search(rf"\b{k}\s*=\s*(-?\d+)", s).group(1))
+    @property
+    def shape_key(self):
+        return (self.block, self.ni, self.nj, self.nvars)
 
-            out.append(dict(block=g("block"), I=g("I"), J=g("J"), nvars=g("NVARS"),
-                            offset=g("OFFSET"), nbytes=g("NBYTES")))
+    @property
+    def count(self):
+        return self.ni * self.nj * self.nvars
+
+
+@dataclass
+class Solution:
+    stem: str
+    precision: str = "Float64"
+    zones: list = field(default_factory=list)
+
+    @property
+    def dtype(self):
+        return np.dtype("<f4") if self.precision == "Float32" else np.dtype("<f8")
+
+    def values(self, zone):
+        """The zone's payload as a float64 array (memory-mapped read)."""
+        if zone.count == 0:
+            return np.zeros(0)
+        m = np.memmap(self.stem + ".dat", dtype=self.dtype, mode="r", offset=zone.offset,
+                      shape=(zone.count,))
+        return np.asarray(m, dtype=np.float64)
+
+
+def _stem(path):
+    root, ext = os.path.splitext(path)
+    return root if ext in (".hdr", ".dat") else path
+
+
+def _fields(line):
+    """KEY=VALUE tokens of a manifest line (spaces around '=' allowed)."""
+    out = {}
+    toks = line.replace("=", " = ").split()
+    for i in range(1, len(toks) - 1):
+        if toks[i] == "=":
+            out[toks[i - 1]] = toks[i + 1]
     return out
 
 
-def load(path):
-    """(fingerprint list, float64 values over all zones, precision)."""
-    hdr, dat = companion(path)
-    prec = precision(hdr)
-    dtype = np.dtype("<f4") if prec == "Float32" else np.dtype("<f8")
-    zs = zones(hdr)
-    keys = [(z["block"], z["I"], z["J"], z["nvars"]) for z in zs]
-    parts = []
-    with open(dat, "rb") as f:
-        for z in zs:
-            f.seek(z["offset"])
-            n = z["nvars"] * z["I"] * z["J"]
-            parts.append(np.frombuffer(f.read(n * dtype.itemsize), dtype=dtype, count=n))
-    vals = np.concatenate(parts).astype(np.float64) if parts else np.zeros(0)
-    return keys, vals, prec
+def read_solution(path):
+    sol = Solution(_stem(path))
+    with open(sol.stem + ".hdr") as f:
+        for raw in f:
+            line = raw.strip()
+            if line.startswith("ZONE"):
+                kv = _fields(line)
+                sol.zones.append(Zone(int(kv["block"]), int(kv["I"]), int(kv["J"]),
+                                      int(kv["NVARS"]), int(kv["OFFSET"])))
+            elif line.startswith("PRECISION"):
+                kv = _fields(line)
+                if "PRECISION" in kv:
+                    sol.precision = kv["PRECISION"]
+    return sol
 
 
 def compare(new, ref, abstol=None, reltol=None, out=print):
-    kn, dn, pn = load(new)
-    kr, dr, pr = load(ref)
-    single = "Float32" in (pn, pr)
-    abstol = (1e-6 if single else 1e-10) if abstol is None else abstol
-    reltol = (1e-5 if single else 1e-8) if reltol is None else reltol
-    if kn != kr:
-        out(f"FAIL: zone structure differs ({len(kn)} vs {len(kr)} zones)")
-        for a, b in zip(kn, kr):
-            if a != b:
-                out(f"   new: {a}\n   ref: {b}")
-                break
-        return 1
-    if dn.size != dr.size:
-        out(f"FAIL: value count differs ({dn.size} vs {dr.size})")
-        return 1
-    d = np.abs(dn - dr)
-    bad = np.nonzero(~(d <= abstol + reltol * np.abs(dr)))[0]
-    if bad.size:
-        i = bad[0]
-        out(f"FAIL: value mismatch at index {i}: {dn[i]:.12g} vs {dr[i]:.12g} (|diff| = {d[i]:.3e})")
-        return 1
-    i = int(np.argmax(d)) if d.size else 0
-    mx = float(d[i]) if d.size else 0.0
-    out(f"PASS: {dn.size} values match within tol (abstol={abstol:g}, reltol={reltol:g}, "
-        f"precision new={pn}/ref={pr}); max |diff| = {mx:.3e}")
-    if mx > 0:
-        out(f"      largest diff at index {i}: {dn[i]:.12g} vs {dr[i]:.12g}")
-    return 0
+    """Compares two solutions; returns the exit status (0 pass, 1 fail) and
+    reports through `out`."""
+    a, b = read_solution(new), read_solution(ref)
+    single = "Float32" in (a.precision, b.precision)
+    d_abs, d_rel = DEFAULT_TOL[single]
+    abstol = d_abs if abstol is None else abstol
+    reltol = d_rel if reltol is None else reltol
+
+    ka = [z.shape_key for z in a.zones]
+    kb = [z.shape_key for z in b.zones]
+    if ka != kb:
+        first = next((i for i, (x, y) in enumerate(zip(ka, kb)) if x != y), min(len(ka), len(kb)))
+        out(f"FAIL structure: {len(ka)} zones against {len(kb)}; first difference at zone {first}: "
+            f"{ka[first] if first < len(ka) else None} against {kb[first] if first < len(kb) else None} "
+            "(block, I, J, NVARS)")
+        return STATUS_FAIL
+
+    seen, worst = 0, (0.0, -1, 0.0, 0.0)
+    for za, zb in zip(a.zones, b.zones):
+        x, y = a.values(za), b.values(zb)
+        diff = np.abs(x - y)
+        over = np.flatnonzero(diff > abstol + reltol * np.abs(y))
+        if over.size:
+            j = int(over[0])
+            out(f"FAIL value: block {za.block} zone value {j} (flat index {seen + j}): new "
+                f"{x[j]:.12g}, ref {y[j]:.12g}, |diff| {diff[j]:.3e} > {abstol:g} + {reltol:g}*|ref|")
+            return STATUS_FAIL
+        if diff.size:
+            j = int(np.nanargmax(diff)) if np.isfinite(diff).any() else 0
+            if diff[j] > worst[0]:
+                worst = (float(diff[j]), seen + j, float(x[j]), float(y[j]))
+        seen += x.size
+
+    out(f"PASS {seen} values of {len(ka)} zones (new {a.precision}, ref {b.precision}; "
+        f"abstol {abstol:g}, reltol {reltol:g}); largest |diff| {worst[0]:.3e}")
+    if worst[0] > 0:
+        out(f"     at flat index {worst[1]}: new {worst[2]:.12g}, ref {worst[3]:.12g}")
+    return STATUS_PASS
 
 
 def main(argv=None):
-    argv = sys.argv[1:] if argv is None else argv
-    if len(argv) < 2:
-        print("usage: compare_solution.py <new> <reference> [abstol] [reltol]")
-        return 2
-    a = float(argv[2]) if len(argv) > 2 else None
-    r = float(argv[3]) if len(argv) > 3 else None
-    return compare(argv[0], argv[1], a, r)
+    args = sys.argv[1:] if argv is None else list(argv)
+    if len(args) < 2:
+        print("usage: python -m multiblockmomentsfv_amd.compare_solution NEW REF [ABSTOL [RELTOL]]")
+        return STATUS_USAGE
+    try:
+        tol = [float(v) for v in args[2:4]]
+    except ValueError:
+        print(f"compare_solution: tolerances must be numbers, got {args[2:4]}")
+        return STATUS_USAGE
+    tol += [None] * (2 - len(tol))
+    return compare(args[0], args[1], tol[0], tol[1])
 
 
 if __name__ == "__main__":

@@ -197,3 +197,42 @@ def test_comparator_semantics(tmp_path):
     assert compare(base, g, out=lambda s: None) == 1
     assert compare(base, g, 1e-3, 1e-3, out=lambda s: None) == 0
     assert main([g]) == 2
+    assert main([g + ".hdr", g + ".dat"]) == 0          # any of hdr / dat / stem
+    # the gate's rule is "fail when |diff| > tol": a NaN on either side passes
+    v2 = np.frombuffer(bytes(d), dtype="<f4").copy()
+    v2[77] = np.nan
+    open(base + ".dat", "wb").write(v2.tobytes())
+    assert compare(base, g, out=lambda s: None) == 0
+    # the tolerance rule is relative to the reference value: 9e-6 relative
+    # passes at the Float32 defaults (1e-6 + 1e-5 |ref|), 2e-5 does not
+    v3 = np.frombuffer(bytes(d), dtype="<f4").copy().astype(np.float64)
+    i = int(np.argmax(np.abs(v3)))
+    for rel, status in ((9e-6, 0), (2e-5, 1)):
+        v4 = v3.copy()
+        v4[i] *= 1 + rel
+        open(base + ".dat", "wb").write(v4.astype("<f4").tobytes())
+        assert compare(base, g, out=lambda s: None) == status, rel
+    # zone structure: a changed NVARS fails regardless of the values
+    hdr = open(g + ".hdr").read().replace("NVARS=10", "NVARS=9", 1)
+    open(base + ".hdr", "w").write(hdr)
+    open(base + ".dat", "wb").write(bytes(d))
+    assert compare(base, g, out=lambda s: None) == 1
+    # no PRECISION entry means Float64: 8-byte values and the tighter defaults
+    h64 = "\n".join(l for l in open(g + ".hdr").read().splitlines() if "PRECISION" not in l)
+    open(base + ".hdr", "w").write(h64.replace("NBYTES=", "X="))
+    v5 = np.frombuffer(bytes(d), dtype="<f4").astype("<f8")
+    with open(base + ".dat", "wb") as f:
+        f.write(v5.tobytes())
+    hd = []
+    off = 0
+    for line in open(base + ".hdr").read().splitlines():
+        if line.startswith("ZONE"):
+            import re as _re
+            n = [int(_re.search(rf"\b{k}=(\d+)", line).group(1)) for k in ("I", "J", "NVARS")]
+            line = _re.sub(r"OFFSET=\d+", f"OFFSET={off}", line)
+            off += 8 * n[0] * n[1] * n[2]
+        hd.append(line)
+    open(base + ".hdr", "w").write("\n".join(hd) + "\n")
+    assert compare(base, base, out=lambda s: None) == 0
+    msgs = []
+    assert compare(base, g, out=msgs.append) == 0 and "new Float64, ref Float32" in msgs[0]
