@@ -144,6 +144,17 @@ int mbm_steps(mbm_ctx *c, double *t, double tend, double cflmax, double cflreg,
  * (float32 != 0) or double. */
 int mbm_output_values(mbm_ctx *c, int block, int float32, void *host);
 
+/* The conversion of mbm_output_values without any device allocation. No
+ * reference counterpart: it is for callers that cannot allocate device
+ * memory (tight output loops, code that forbids hipMalloc/hipFree). The
+ * device-side values are staged in a static per-GPU buffer of
+ * MBM_OUTPUT_SCRATCH_BYTES, held until the process ends; a block whose output
+ * ((nx+2)(ny+2) points x 10 values x 4 or 8 bytes) exceeds it is rejected
+ * with MBM_E_ARG. Calls on one GPU share that buffer: calls that overlap in
+ * time on one GPU (other threads, other streams) race. */
+#define MBM_OUTPUT_SCRATCH_BYTES (64 * 1024 * 1024)
+int mbm_output_values_noalloc(mbm_ctx *c, int block, int float32, void *host);
+
 /* Number of interior cells (tools.c:87-95) and of cells incl. ghosts. */
 long long mbm_interior_cells(const mbm_ctx *c);
 int mbm_block_dims(const mbm_ctx *c, int block, int *nx, int *ny);

@@ -25,6 +25,7 @@
 #include <vector>
 #include <string>
 #include <algorithm>
+#include <mutex>
 
 #include "mbmfv.h"
 #include "mbmfv_host.h"
@@ -50,6 +51,36 @@ int set_err(int code, const char *fmt, ...)
 using namespace mbmi;
 
 using namespace mbm;
+
+/* Staging buffer of mbm_output_values_noalloc: one copy per GPU, held for
+ * the life of the process. On the host the name `scratch` is a stand-in, not
+ * a device address: host code uses only the per-device address that
+ * hipGetSymbolAddress returns (resolved once per device, with that device
+ * current, when the first context is created on it). Every call on one GPU
+ * shares the buffer, so calls that overlap on one GPU race. */
+static __device__ __align__(256) unsigned char scratch[MBM_OUTPUT_SCRATCH_BYTES];
+
+namespace mbmi {
+constexpr int kMaxScratchDevices = 64;
+void *g_scratch[kMaxScratchDevices];
+std::mutex g_scratch_mu;
+
+/* device address of `scratch` on the current device dev */
+int resolve_scratch(int dev)
+{
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return set_err(MBM_E_NODEV, "device %d outside [0,%d)", dev, kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[dev]) return MBM_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return set_err(MBM_E_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                   hipGetErrorString(e));
+  g_scratch[dev] = p;
+  return MBM_OK;
+}
+} // namespace mbmi
 
 /* waves per SIMD the stage kernel is compiled for: 2 (256 registers per
  * wave, 220 used) lets one wave's memory waits overlap the other's FP64 work */
@@ -1060,6 +1091,7 @@ mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_f
   c->deck_first.assign(deck_first, deck_first + ndecks + 1);
   c->deck_par.assign(deck_par, deck_par + ndecks);
   if (!hipok(hipGetDevice(&c->device), "hipGetDevice")) return fail(rc);
+  if ((rc = resolve_scratch(c->device))) return fail(rc);
   for (int d = 0; d < ndecks; d++)
     for (int k = deck_first[d]; k < deck_first[d + 1]; k++) {
       BlockDev b{};
@@ -1630,15 +1662,19 @@ extern "C" int mbm_steps(mbm_ctx *c, double *t, double tend, double cflmax, doub
   return MBM_OK;
 }
 
-extern "C" int mbm_output_values(mbm_ctx *c, int block, int float32, void *host)
+namespace {
+/* bytes of the Output.c conversion of one block: (nx+2)(ny+2) points of 10 values */
+size_t output_bytes(const mbm_ctx *c, int block, int float32)
 {
-  int rc = check_lb(c, 0, block);
-  if (rc) return rc;
+  const size_t npt = (size_t)(c->desc[block].nx + 2) * (c->desc[block].ny + 2);
+  return npt * 10 * (float32 ? sizeof(float) : sizeof(double));
+}
+
+/* converts DATA of one block into the device buffer d and copies it to host */
+hipError_t output_through(mbm_ctx *c, int block, int float32, void *d, void *host)
+{
   const int nx = c->desc[block].nx, ny = c->desc[block].ny;
   const size_t npt = (size_t)(nx + 2) * (ny + 2);
-  const size_t bytes = npt * 10 * (float32 ? sizeof(float) : sizeof(double));
-  void *d = nullptr;
-  HIPCHK(hipMalloc(&d, bytes));
   const int nblk = (int)((npt + 255) / 256);
   if (float32)
     hipLaunchKernelGGL(output_kernel<float>, dim3(nblk), dim3(256), 0, c->stream,
@@ -1647,9 +1683,40 @@ extern "C" int mbm_output_values(mbm_ctx *c, int block, int float32, void *host)
     hipLaunchKernelGGL(output_kernel<double>, dim3(nblk), dim3(256), 0, c->stream,
                        c->layer[c->role[MBM_L_DATA]], c->dblk, block, nx, ny, (double *)d);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(host, d, output_bytes(c, block, float32), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
+}
+} // namespace
+
+extern "C" int mbm_output_values(mbm_ctx *c, int block, int float32, void *host)
+{
+  int rc = check_lb(c, 0, block);
+  if (rc) return rc;
+  if (!host) return set_err(MBM_E_ARG, "NULL host buffer");
+  void *d = nullptr;
+  HIPCHK(hipMalloc(&d, output_bytes(c, block, float32)));
+  const hipError_t e = output_through(c, block, float32, d, host);
   (void)hipFree(d);
+  if (e != hipSuccess) return set_err(MBM_E_HIP, "output conversion failed: %s", hipGetErrorString(e));
+  return MBM_OK;
+}
+
+extern "C" int mbm_output_values_noalloc(mbm_ctx *c, int block, int float32, void *host)
+{
+  int rc = check_lb(c, 0, block);
+  if (rc) return rc;
+  if (!host) return set_err(MBM_E_ARG, "NULL host buffer");
+  const size_t bytes = output_bytes(c, block, float32);
+  if (bytes > (size_t)MBM_OUTPUT_SCRATCH_BYTES)
+    return set_err(MBM_E_ARG,
+                   "block %d: its output (%zu bytes) exceeds the %d-byte static staging buffer of "
+                   "mbm_output_values_noalloc; use mbm_output_values", block, bytes,
+                   MBM_OUTPUT_SCRATCH_BYTES);
+  void *d = c->device >= 0 && c->device < kMaxScratchDevices ? g_scratch[c->device] : nullptr;
+  if (!d) return set_err(MBM_E_STATE, "static staging buffer not resolved on device %d", c->device);
+  const hipError_t e = output_through(c, block, float32, d, host);
   if (e != hipSuccess) return set_err(MBM_E_HIP, "output conversion failed: %s", hipGetErrorString(e));
   return MBM_OK;
 }
