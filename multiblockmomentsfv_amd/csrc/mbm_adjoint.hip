@@ -509,8 +509,8 @@ int adj_setup(mbm_ctx *c)
   int rc;
   for (int k = 0; k < 3; k++) if ((rc = alloc_layer(c, &A->g[k]))) return rc;
   if ((rc = alloc_layer(c, &A->gpy))) return rc;
-  HIPCHK(hipMalloc(&A->slots, (size_t)6 * c->nb * sizeof(unsigned long long)));
-  HIPCHK(hipMemsetAsync(A->slots, 0, (size_t)6 * c->nb * sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMalloc(&A->slots, (size_t)2 * SLOTS_PER_BLOCK * c->nb * sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(A->slots, 0, (size_t)2 * SLOTS_PER_BLOCK * c->nb * sizeof(unsigned long long), c->stream));
   /* adj_x: 58 owned columns per wavefront (lanes 3..60), every row of the
    * block. adj_y: one column per lane over R rows of 1..ly-2, 64 columns per
    * wavefront, a block's remaining columns packed several row runs to a
@@ -697,7 +697,7 @@ int step_buffers(mbm_ctx *c, const double *D, double *T, double *N, double dt)
   mbm_adj_state *A = c->adj;
   int rc;
   if ((rc = stage1_buffers(c, D, T, dt))) return rc;
-  if ((rc = enqueue_stage(c, 2, T, D, N, dt, A->slots + 3 * c->nb))) return rc;
+  if ((rc = enqueue_stage(c, 2, T, D, N, dt, A->slots + SLOTS_PER_BLOCK * c->nb))) return rc;
   return fill_buffer(c, N);
 }
 

@@ -26,6 +26,7 @@
 #include <string>
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "mbmfv.h"
 #include "mbmfv_host.h"
@@ -95,28 +96,45 @@ int resolve_scratch(int dev)
 #define MBM_STAGE2_DOWN 1
 #endif
 
-template <int STAGE, int LIM, bool DOWN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MBM_STAGE_WAVES, MBM_STAGE_WAVES)))
-stage_kernel(const StageArgs A)
+/* the CFL parts of one tile: face-speed and production maxima (NaN
+ * ignored), and 1 + the scan position of its last NaN production CFL */
+struct TileOut {
+  double mxx, mxy, mxp;
+  unsigned long long pnan;
+  int blk;
+};
+
+/* One pass of a wavefront over its tile: one Heun stage of Update
+ * (Main.c:297-396) on a strip of 60 columns and a run of rows. sm: the
+ * wavefront's lane-private LDS slots (the reconstructed state of the current
+ * row on the face ahead of the march -- the upper state when marching up,
+ * the lower one when marching down -- and the initial values of the next two
+ * rows, by row parity), which keep that state out of registers.
+ *
+ * Returns 0 for a tile with nothing to do (padding, or its deck is done).
+ * SLOWOK false: the fast forms only; returns 2 if some row's stencil held
+ * non-finite data or some face density came within rounding of zero, else 1.
+ * SLOWOK true: rows with non-finite data use the reference's forms
+ * (reconstruct_ref, face_ref), and a face whose density is exactly zero in
+ * the reference's arithmetic is 0/0. A tile whose fast pass returned true is
+ * run again with SLOWOK true; that pass rewrites every stored cell and the
+ * CFL parts. The fast pass has no branches for these cases: its flags are
+ * lane masks accumulated in scalar registers. */
+template <int STAGE, int LIM, bool DOWN, bool SLOWOK>
+__device__ __forceinline__ int tile_pass(const StageArgs &A, int tix, double (*sm)[NV][64], TileOut &o)
 {
-  /* the production CFL (stage 1) is folded by the upward march only */
-  static_assert(!DOWN || STAGE == 2, "downward march is stage 2 only");
-  const int lane = threadIdx.x & 63;
-  /* wave-uniform: tile, block and deck data are loaded into scalar registers */
-  const int tix = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6));
-  if (tix >= A.ntiles) return;
+  /* tile, block and deck data (wave-uniform, scalar registers); each pass
+   * loads them itself, so that nothing of the first pass is kept for the
+   * second */
   const TileDev T = A.tiles[tix];
-  if (T.blk < 0) return;                 /* padding of the XCD ordering */
+  if (T.blk < 0) return 0;               /* padding of the XCD ordering */
   const BlockDev B = A.blocks[T.blk];
-  const int lx = B.lx, ly = B.ly;
-  const int a = T.a0 + lane;
-  const int ac = a < lx ? a : lx - 1;
   const double *in = A.in, *in0 = A.in0;
   double *out = A.out;
   double dt = A.dt, dtkn = A.dtkn, force = A.force;
   if (A.decks) {                       /* device-controlled: per-deck dt and buffers */
     const DeckState &S = A.decks[B.deck];
-    if (S.done) return;
+    if (S.done) return 0;
     dt = S.dt;
     dtkn = S.dt / S.kn;
     force = S.force;
@@ -126,6 +144,10 @@ stage_kernel(const StageArgs A)
     in0 = data;
     out = STAGE == 1 ? A.tmp : newb;
   }
+  const int lane = threadIdx.x & 63;
+  const int lx = B.lx, ly = B.ly;
+  const int a = T.a0 + lane;
+  const int ac = a < lx ? a : lx - 1;
   /* block bases (uniform) and per-lane byte offsets within a block; the host
    * guarantees that a block spans less than 2^31 bytes */
   const long long nel = (long long)B.npos * NV * B.pitch;
@@ -143,44 +165,94 @@ stage_kernel(const StageArgs A)
   const bool col_face = (a >= 1 && a <= lx - 3);
   const bool xface_ok = col_face && lane >= 1 && lane <= 61;
   const bool st_col = (lane >= 2 && lane <= 61) && a >= MBC && a <= lx - 1 - MBC;
+#define TUP(n) sm[0][n][lane]
+#define INIT(row, n) sm[1 + ((row) & 1)][n][lane]
 
+  bool anyslow = false;
   double mxx = 0.0, mxy = 0.0, mxp = 0.0;
+  unsigned long long pnan = 0ull;
   double U[NV], U0[NV], dummy[NV];
   double cp = 0.0;
 
   auto row_of = [&](int r) { return r < 0 ? 0 : (r > ly - 1 ? ly - 1 : r); };
 
-  /* lane-private LDS slots for the state carried between rows: the
-   * reconstructed state of the current row on the face ahead of the march
-   * (slot 0: upper state when marching up, lower state when marching down)
-   * and the initial values of the next two rows (slots 1, 2 by row parity);
-   * keeps them out of registers */
-  __shared__ double sm[WAVES_PER_WG][3][NV][64];
-  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-#define TUP(n) sm[wv][0][n][lane]
-#define INIT(row, n) sm[wv][1 + ((row) & 1)][n][lane]
+  /* Non-finite data (wave-uniform bookkeeping, scalar registers only):
+   * nfrows has bit 0 set when the last projected row holds a non-finite
+   * relaxed primitive in some lane, bit 1 for the row before, ...; a face
+   * whose stencil rows have a bit set is computed in the reference's form
+   * (reconstruct_ref, face_ref). zcarry marks the lanes whose carried face
+   * state has a density that is exactly zero in the reference's arithmetic. */
+  unsigned nfrows = 0;
+  unsigned long long zcarry = 0ull;
+  /* lanes whose fast face density came within rounding of zero: the exact
+   * decision (zero_density_ref) is left to the second pass */
+  unsigned long long mzero = 0ull;
+  auto note_maybe_zero = [&](bool c) { mzero |= __builtin_amdgcn_ballot_w64(c); };
+  /* the form a face takes: reference's only in the second pass */
+  auto slow_rows = [&](bool nonfinite) {
+    anyslow = anyslow || nonfinite;
+    return SLOWOK && nonfinite;
+  };
+  auto note_row = [&](bool bad) {
+    nfrows = (nfrows << 1) | (__builtin_amdgcn_ballot_w64(bad) != 0ull ? 1u : 0u);
+  };
+  /* a NaN production CFL of cell (a, row) at ring >= 1: 1 + its position in
+   * the reference's scan order (i outer, j inner) for the controller */
+  auto note_prod = [&](int row) {
+    if (__builtin_expect(cp != cp, 0) && col_prod && row >= 1 && row <= ly - 2) {
+      const unsigned long long q = (unsigned long long)(a - 1) * (ly - 2) + row;
+      pnan = q > pnan ? q : pnan;
+    }
+  };
 
   double Pa[NV], Pb[NV], Pc[NV], Pn[NV], acc[NV];
 
   /* x-faces of the current row (primitives in Pc): acc += F(i-1/2) - F(i+1/2) */
-  auto xfaces = [&]() {
+  auto xfaces = [&](bool slow) {
     double L[NV], Rr[NV], F[NV], s;
+    if (__builtin_expect(slow, 0)) {
 #pragma unroll
-    for (int g = 0; g < NV; g += 3) {
-      double e1[3], e2[3], sx[3];
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        e1[k] = Pc[g + k] - from_left(Pc[g + k]);
-        e2[k] = from_right(Pc[g + k]) - Pc[g + k];
+      for (int n = 0; n < NV; n++) {
+        double lo, hi;
+        reconstruct_ref<LIM>(Pc[n], Pc[n] - from_left(Pc[n]), from_right(Pc[n]) - Pc[n], B.dx, lo, hi);
+        L[n] = hi;
+        Rr[n] = from_right(lo);
       }
-      limit3<LIM>(e1, e2, sx);
+      face_ref<0>(L, Rr, dtdx, F, s);
+    } else {
+      double d10 = 0.0, d20 = 0.0, sx0 = 0.0;
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        L[g + k] = Pc[g + k] + 0.5 * sx[k];
-        Rr[g + k] = from_right(Pc[g + k] - 0.5 * sx[k]);
+      for (int g = 0; g < NV; g += 3) {
+        double e1[3], e2[3], sx[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          e1[k] = Pc[g + k] - from_left(Pc[g + k]);
+          e2[k] = from_right(Pc[g + k]) - Pc[g + k];
+        }
+        limit3<LIM>(e1, e2, sx);
+        if (g == 0) {
+          d10 = e1[0];
+          d20 = e2[0];
+          sx0 = sx[0];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          L[g + k] = Pc[g + k] + 0.5 * sx[k];
+          Rr[g + k] = from_right(Pc[g + k] - 0.5 * sx[k]);
+        }
+      }
+      face<0>(L, Rr, dtdx, F, s);
+      if constexpr (SLOWOK) {
+        bool zlo, zhi;
+        zero_density_ref<LIM>(Pc[0], d10, d20, B.dx, Pc[0] - 0.5 * sx0, L[0], zlo, zhi);
+        /* face a+1/2: upper state of lane a, lower state of lane a+1 */
+        const unsigned long long zf = __builtin_amdgcn_ballot_w64(zhi) | (__builtin_amdgcn_ballot_w64(zlo) >> 1);
+        if (__builtin_expect(zf != 0ull, 0))
+          if ((zf >> lane) & 1ull) nan_face(F);
+      } else {
+        note_maybe_zero(maybe_zero_density(Pc[0], Pc[0] - 0.5 * sx0, L[0]));
       }
     }
-    face<0>(L, Rr, dtdx, F, s);
     if (xface_ok) fold_max(mxx, s);
 #pragma unroll
     for (int n = 0; n < NV; n++) acc[n] += from_left(F[n]) - F[n];
@@ -191,56 +263,116 @@ stage_kernel(const StageArgs A)
     load9b(rin, pb, roff, Ud);
     if constexpr (STAGE == 2) load9b(rin0, pb, roff, U0d);
   };
+  /* one row of the block: the fast face states, or the reference's
+   * (slow), of the row with primitives pc between pm (below) and pp; zlo,
+   * zhi: the reference's density of the lower/upper state is exactly 0.
+   * CARRY 1: the upper state goes to the lane's LDS slot instead of hi,
+   * CARRY 2: the lower state does (it is carried to the next row's face) */
+  auto ystates = [&](auto carry, const double *pm, const double *pc, const double *pp, bool slow,
+                     double *lo, double *hi, bool &zlo, bool &zhi) {
+    constexpr int CARRY = decltype(carry)::value;
+    auto put = [&](int n, double l, double h) {
+      if constexpr (CARRY == 1) { TUP(n) = h; lo[n] = l; }
+      else if constexpr (CARRY == 2) { TUP(n) = l; hi[n] = h; }
+      else { lo[n] = l; hi[n] = h; }
+    };
+    if (__builtin_expect(slow, 0)) {
+#pragma unroll
+      for (int n = 0; n < NV; n++) {
+        double l, h;
+        reconstruct_ref<LIM>(pc[n], pc[n] - pm[n], pp[n] - pc[n], B.dy, l, h);
+        if (n == 0) {
+          zlo = l == 0.0;
+          zhi = h == 0.0;
+        }
+        put(n, l, h);
+      }
+      return;
+    }
+    double d10 = 0.0, d20 = 0.0, lo0 = 0.0, hi0 = 0.0;
+#pragma unroll
+    for (int g = 0; g < NV; g += 3) {
+      double e1[3], e2[3], sy[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        e1[k] = pc[g + k] - pm[g + k];
+        e2[k] = pp[g + k] - pc[g + k];
+      }
+      limit3<LIM>(e1, e2, sy);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double l = pc[g + k] - 0.5 * sy[k], h = pc[g + k] + 0.5 * sy[k];
+        if (g + k == 0) {
+          d10 = e1[0];
+          d20 = e2[0];
+          lo0 = l;
+          hi0 = h;
+        }
+        put(g + k, l, h);
+      }
+    }
+    if constexpr (SLOWOK) {
+      zero_density_ref<LIM>(pc[0], d10, d20, B.dy, lo0, hi0, zlo, zhi);
+    } else {
+      zlo = zhi = false;
+      note_maybe_zero(maybe_zero_density(pc[0], lo0, hi0));
+    }
+  };
+  using both_ = std::integral_constant<int, 0>;
+  using up_ = std::integral_constant<int, 1>;
+  using down_ = std::integral_constant<int, 2>;
+  /* the y-face between lower state Lo (from below) and upper state Up; znan:
+   * lanes whose face is 0/0 in the reference */
+  auto yface = [&](const double *Lo, const double *Up, bool slow, unsigned long long znan, double *G,
+                   double &s) {
+    if (__builtin_expect(slow, 0)) face_ref<1>(Lo, Up, dtdy, G, s);
+    else face<1>(Lo, Up, dtdy, G, s);
+    if constexpr (SLOWOK) {
+      if (__builtin_expect(znan != 0ull, 0))
+        if ((znan >> lane) & 1ull) nan_face(G);
+    }
+  };
 
   if constexpr (!DOWN) {
     /* ---- warm-up: rows r0-2 .. r0+1 ---- */
     const int r0 = T.r0;
     {
       double Ir[NV];
+      bool bad;
       int rr = row_of(r0 - 2);
       load9b(rin, pb, cin + rowb(rr), U);
-      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
+      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp, bad);
+      note_row(bad);
       rr = row_of(r0 - 1);
       load9b(rin, pb, cin + rowb(rr), U);
-      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp);
+      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp, bad);
+      note_row(bad);
       rr = row_of(r0);
       load9b(rin, pb, cin + rowb(rr), U);
       if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb(rr), U0);
-      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp);
+      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp, bad);
+      note_row(bad);
+      note_prod(rr);
       if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
       rr = row_of(r0 + 1);
       load9b(rin, pb, cin + rowb(rr), U);
       if constexpr (STAGE == 2) load9b(rin0, pb, cin + rowb(rr), U0);
-      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp);
+      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp, bad);
+      note_row(bad);
+      note_prod(rr);
       if (STAGE == 1 && col_prod && rr >= 1 && rr <= ly - 2) fold_max(mxp, cp);
 #pragma unroll
       for (int n = 0; n < NV; n++) INIT(r0 + 1, n) = Ir[n];
 
-      double Tprev[NV], Bot[NV], G[NV], s;
-      /* slopes in groups of three, exactly as in the march (same rounding) */
-#pragma unroll
-      for (int g = 0; g < NV; g += 3) {
-        double e1[3], e2[3], sp[3], sc[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          e1[k] = Pb[g + k] - Pa[g + k];
-          e2[k] = Pc[g + k] - Pb[g + k];
-        }
-        limit3<LIM>(e1, e2, sp);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          e1[k] = Pc[g + k] - Pb[g + k];
-          e2[k] = Pn[g + k] - Pc[g + k];
-        }
-        limit3<LIM>(e1, e2, sc);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          Tprev[g + k] = Pb[g + k] + 0.5 * sp[k];
-          TUP(g + k) = Pc[g + k] + 0.5 * sc[k];
-          Bot[g + k] = Pc[g + k] - 0.5 * sc[k];
-        }
-      }
-      face<1>(Tprev, Bot, dtdy, G, s);
+      /* face r0-1/2 from the states of rows r0-1 and r0 (same rounding as
+       * the march computes them) */
+      const bool slow = slow_rows((nfrows & 0xfu) != 0u);
+      double lo[NV], Tprev[NV], Bot[NV], G[NV], s;
+      bool z0, zp, zb, zt;
+      ystates(both_{}, Pa, Pb, Pc, slow, lo, Tprev, z0, zp);
+      ystates(up_{}, Pb, Pc, Pn, slow, Bot, nullptr, zb, zt);
+      zcarry = __builtin_amdgcn_ballot_w64(zt);
+      yface(Tprev, Bot, slow, __builtin_amdgcn_ballot_w64(zp) | __builtin_amdgcn_ballot_w64(zb), G, s);
       if (col_face && r0 - 1 >= 1 && r0 - 1 <= ly - 3) fold_max(mxy, s);
 #pragma unroll
       for (int n = 0; n < NV; n++) acc[n] += G[n];
@@ -248,7 +380,8 @@ stage_kernel(const StageArgs A)
 
     /* ---- march: row r, with the input of row r+2 in (Uc, U0c) ---- */
     auto march_row = [&](int r, const double *Uc, const double *U0c) {
-      xfaces();
+      xfaces(slow_rows(((nfrows >> 1) & 1u) != 0u));
+      if constexpr (SLOWOK) fetch(r + 2, U, U0);
       __builtin_amdgcn_sched_barrier(0);
 
       /* row r+2: projection */
@@ -256,34 +389,26 @@ stage_kernel(const StageArgs A)
       const int rr = r + 2;
       {
         double I2[NV];
-        project<STAGE, true>(Uc, U0c, col_ring3 || rr == ly - 1, dt, dtkn, force, P2, I2, cp);
+        bool bad;
+        project<STAGE, true>(Uc, U0c, col_ring3 || rr == ly - 1, dt, dtkn, force, P2, I2, cp, bad);
+        note_row(bad);
 #pragma unroll
         for (int n = 0; n < NV; n++) INIT(rr, n) = I2[n];
       }
+      note_prod(rr);
       if (STAGE == 1 && col_prod && rr <= ly - 2) fold_max(mxp, cp);
       __builtin_amdgcn_sched_barrier(0);
 
-      /* y-face r+1/2 */
+      /* y-face r+1/2: upper state of row r (carried), lower state of row r+1 */
       {
+        const bool slow = slow_rows((nfrows & 0xfu) != 0u);
         double Tu[NV], Bn[NV], G[NV], s;
+        bool zb, zt;
 #pragma unroll
         for (int n = 0; n < NV; n++) Tu[n] = TUP(n);
-#pragma unroll
-        for (int g = 0; g < NV; g += 3) {
-          double e1[3], e2[3], sy[3];
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            e1[k] = Pn[g + k] - Pc[g + k];
-            e2[k] = P2[g + k] - Pn[g + k];
-          }
-          limit3<LIM>(e1, e2, sy);
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            TUP(g + k) = Pn[g + k] + 0.5 * sy[k];
-            Bn[g + k] = Pn[g + k] - 0.5 * sy[k];
-          }
-        }
-        face<1>(Tu, Bn, dtdy, G, s);
+        ystates(up_{}, Pc, Pn, P2, slow, Bn, nullptr, zb, zt);
+        yface(Tu, Bn, slow, zcarry | __builtin_amdgcn_ballot_w64(zb), G, s);
+        zcarry = __builtin_amdgcn_ballot_w64(zt);
         if (col_face) fold_max(mxy, s);
         if (st_col && r >= MBC && r <= ly - 1 - MBC) {
           const unsigned ob = cout + rowb(r);
@@ -302,8 +427,9 @@ stage_kernel(const StageArgs A)
       }
     };
     for (int r = r0; r <= T.r1; r++) {
-      /* row r+2 is loaded while the x-faces of row r are computed */
-      fetch(r + 2, U, U0);
+      /* row r+2 is loaded while the x-faces of row r are computed (in the
+       * second pass after them: fewer registers live) */
+      if constexpr (!SLOWOK) fetch(r + 2, U, U0);
       march_row(r, U, U0);
     }
   } else {
@@ -311,49 +437,39 @@ stage_kernel(const StageArgs A)
     const int r1 = T.r1;
     {
       double Ir[NV];
+      bool bad;
       int rr = row_of(r1 + 2);
       load9b(rin, pb, cin + rowb(rr), U);
-      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp);
+      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pa, dummy, cp, bad);
+      note_row(bad);
       rr = row_of(r1 + 1);
       load9b(rin, pb, cin + rowb(rr), U);
-      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp);
+      project<STAGE, false>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pb, dummy, cp, bad);
+      note_row(bad);
       rr = row_of(r1);
       load9b(rin, pb, cin + rowb(rr), U);
       load9b(rin0, pb, cin + rowb(rr), U0);
-      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp);
+      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pc, acc, cp, bad);
+      note_row(bad);
+      note_prod(rr);
       rr = row_of(r1 - 1);
       load9b(rin, pb, cin + rowb(rr), U);
       load9b(rin0, pb, cin + rowb(rr), U0);
-      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp);
+      project<STAGE, true>(U, U0, col_ring3 || rr == 0 || rr == ly - 1, dt, dtkn, force, Pn, Ir, cp, bad);
+      note_row(bad);
+      note_prod(rr);
 #pragma unroll
       for (int n = 0; n < NV; n++) INIT(r1 - 1, n) = Ir[n];
 
-      /* face r1+1/2 from the slopes of rows r1+1 and r1, grouped as in the
-       * march, so that it equals the face the tile above computes */
-      double Tl[NV], Bu[NV], G[NV], s;
-#pragma unroll
-      for (int g = 0; g < NV; g += 3) {
-        double e1[3], e2[3], sh[3], sc[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          e1[k] = Pb[g + k] - Pc[g + k];
-          e2[k] = Pa[g + k] - Pb[g + k];
-        }
-        limit3<LIM>(e1, e2, sh);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          e1[k] = Pc[g + k] - Pn[g + k];
-          e2[k] = Pb[g + k] - Pc[g + k];
-        }
-        limit3<LIM>(e1, e2, sc);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          Bu[g + k] = Pb[g + k] - 0.5 * sh[k];
-          Tl[g + k] = Pc[g + k] + 0.5 * sc[k];
-          TUP(g + k) = Pc[g + k] - 0.5 * sc[k];
-        }
-      }
-      face<1>(Tl, Bu, dtdy, G, s);
+      /* face r1+1/2 from the states of rows r1+1 and r1, as the march
+       * computes them, so that it equals the face the tile above computes */
+      const bool slow = slow_rows((nfrows & 0xfu) != 0u);
+      double Bu[NV], hi[NV], Tl[NV], G[NV], s;
+      bool zb, z0, zlc, zt;
+      ystates(both_{}, Pc, Pb, Pa, slow, Bu, hi, zb, z0);
+      ystates(down_{}, Pn, Pc, Pb, slow, nullptr, Tl, zlc, zt);
+      zcarry = __builtin_amdgcn_ballot_w64(zlc);
+      yface(Tl, Bu, slow, __builtin_amdgcn_ballot_w64(zt) | __builtin_amdgcn_ballot_w64(zb), G, s);
       if (col_face) fold_max(mxy, s);
 #pragma unroll
       for (int n = 0; n < NV; n++) acc[n] -= G[n];
@@ -362,7 +478,8 @@ stage_kernel(const StageArgs A)
     /* ---- march down: row r, with the input of row r-2 in (Uc, U0c); a cell
      * accumulates init - G(j+1/2) + (F(i-1/2) - F(i+1/2)) + G(j-1/2) ---- */
     auto march_row = [&](int r, const double *Uc, const double *U0c) {
-      xfaces();
+      xfaces(slow_rows(((nfrows >> 1) & 1u) != 0u));
+      if constexpr (SLOWOK) fetch(r - 2, U, U0);
       __builtin_amdgcn_sched_barrier(0);
 
       /* row r-2 (clamped to ring 3 below r = 1): projection */
@@ -370,33 +487,25 @@ stage_kernel(const StageArgs A)
       {
         const int rr = row_of(r - 2);
         double I2[NV];
-        project<STAGE, true>(Uc, U0c, col_ring3 || rr == 0, dt, dtkn, force, P2, I2, cp);
+        bool bad;
+        project<STAGE, true>(Uc, U0c, col_ring3 || rr == 0, dt, dtkn, force, P2, I2, cp, bad);
+        note_row(bad);
+        note_prod(r - 2);
 #pragma unroll
         for (int n = 0; n < NV; n++) INIT(r - 2, n) = I2[n];
       }
       __builtin_amdgcn_sched_barrier(0);
 
-      /* y-face r-1/2 */
+      /* y-face r-1/2: upper state of row r-1, lower state of row r (carried) */
       {
+        const bool slow = slow_rows((nfrows & 0xfu) != 0u);
         double Bu[NV], Tl[NV], G[NV], s;
+        bool zb, zt;
 #pragma unroll
         for (int n = 0; n < NV; n++) Bu[n] = TUP(n);
-#pragma unroll
-        for (int g = 0; g < NV; g += 3) {
-          double e1[3], e2[3], sy[3];
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            e1[k] = Pn[g + k] - P2[g + k];
-            e2[k] = Pc[g + k] - Pn[g + k];
-          }
-          limit3<LIM>(e1, e2, sy);
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            TUP(g + k) = Pn[g + k] - 0.5 * sy[k];
-            Tl[g + k] = Pn[g + k] + 0.5 * sy[k];
-          }
-        }
-        face<1>(Tl, Bu, dtdy, G, s);
+        ystates(down_{}, P2, Pn, Pc, slow, nullptr, Tl, zb, zt);
+        yface(Tl, Bu, slow, zcarry | __builtin_amdgcn_ballot_w64(zt), G, s);
+        zcarry = __builtin_amdgcn_ballot_w64(zb);
         if (col_face && r - 1 >= 1) fold_max(mxy, s);
         if (st_col && r >= MBC && r <= ly - 1 - MBC) {
           const unsigned ob = cout + rowb(r);
@@ -415,24 +524,53 @@ stage_kernel(const StageArgs A)
       }
     };
     for (int r = r1; r >= T.r0; r--) {
-      /* row r-2 is loaded while the x-faces of row r are computed */
-      fetch(r - 2, U, U0);
+      /* row r-2 is loaded while the x-faces of row r are computed (in the
+       * second pass after them) */
+      if constexpr (!SLOWOK) fetch(r - 2, U, U0);
       march_row(r, U, U0);
     }
   }
 #undef TUP
 #undef INIT
+  o.mxx = mxx;
+  o.mxy = mxy;
+  o.mxp = mxp;
+  o.pnan = pnan;
+  o.blk = T.blk;
+  return (anyslow || mzero != 0ull) ? 2 : 1;
+}
+
+template <int STAGE, int LIM, bool DOWN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MBM_STAGE_WAVES, MBM_STAGE_WAVES)))
+stage_kernel(const StageArgs A)
+{
+  /* the production CFL (stage 1) is folded by the upward march only */
+  static_assert(!DOWN || STAGE == 2, "downward march is stage 2 only");
+  const int lane = threadIdx.x & 63;
+  const int tix = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6));
+  if (tix >= A.ntiles) return;
+  __shared__ double sm[WAVES_PER_WG][3][NV][64];
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  TileOut o;
+  const int pass = tile_pass<STAGE, LIM, DOWN, false>(A, tix, sm[wv], o);
+  if (pass == 0) return;
+  if (__builtin_expect(pass == 2, 0)) tile_pass<STAGE, LIM, DOWN, true>(A, tix, sm[wv], o);
+  double mxx = o.mxx, mxy = o.mxy, mxp = o.mxp;
+  const unsigned long long pnan = o.pnan;
 
   /* ---- CFL maxima of this wavefront ---- */
   mxx = wave_max(mxx);
   mxy = wave_max(mxy);
   if (STAGE == 1) mxp = wave_max(mxp);
   if (lane == 0) {
-    unsigned long long *s = A.slots + 3 * T.blk;
+    unsigned long long *s = A.slots + SLOTS_PER_BLOCK * o.blk;
     atomicMax(s + 0, (unsigned long long)__double_as_longlong(mxx));
     atomicMax(s + 1, (unsigned long long)__double_as_longlong(mxy));
     if (STAGE == 1) atomicMax(s + 2, (unsigned long long)__double_as_longlong(mxp));
   }
+  /* the last NaN production CFL of this wavefront (rare) */
+  if (__builtin_expect(__builtin_amdgcn_ballot_w64(pnan != 0ull) != 0ull, 0))
+    if (pnan) atomicMax(A.slots + SLOTS_PER_BLOCK * o.blk + 3, pnan);
 }
 
 /* SetBoundary as a gather over the precomputed ghost map (Boundary.c:173-193).
@@ -469,29 +607,75 @@ __global__ void __launch_bounds__(256) fill_kernel(double *layer, const FillRec 
   for (int m = 0; m < NV; m++) dst[(long long)m * g.dplane] = ((g.sign >> m) & 1) ? -v[m] : v[m];
 }
 
-/* Steps controller of every deck (Main.c:202-216) after one TimeUpdate:
- * combined CFL (Main.c:395, :268-282), accept/reject, dt update, clamp to
- * tend, parity flip instead of the New->Data copy; clears the slots. */
-__global__ void control_kernel(DeckState *decks, unsigned long long *slots,
-                               const BlockDev *blocks, const int *first, int ndecks, int nb,
-                               RecordDev *rec)
+/* Production CFL of block B (stage STAGE) in the reference's fold when a NaN
+ * occurred: Main.c:325-326 folds CFL_prod = max(CFL_prod, new), which keeps
+ * the running value only while new is a number, so the result is the
+ * maximum over the cells after the last NaN in the i-outer, j-inner scan
+ * (pnan = 1 + its position), NaN when that NaN is the last cell. Recomputed
+ * from the stage input by the calling wavefront (64 lanes, all lanes call). */
+template <int STAGE>
+__device__ double prod_after_nan(const double *layer, const BlockDev &B, unsigned long long pnan,
+                                 double dt, double dtkn, double force)
 {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= ndecks) return;
+  const unsigned long long ny2 = (unsigned long long)(B.ly - 2);
+  const unsigned long long ncell = (unsigned long long)(B.lx - 2) * ny2;
+  if (pnan >= ncell) return __builtin_nan("");
+  double m = -HUGE_VAL;
+  for (unsigned long long q = pnan + (threadIdx.x & 63); q < ncell; q += 64) {
+    const int a = 1 + (int)(q / ny2), b = 1 + (int)(q % ny2);
+    const double *p = layer + cell_elem(B, b, a);
+    double U[NV], P[NV], I[NV], cp;
+    bool bad;
+    for (int n = 0; n < NV; n++) U[n] = p[n * B.plane];
+    project<STAGE, true>(U, U, false, dt, dtkn, force, P, I, cp, bad);
+    if (cp > m) m = cp;
+  }
+  return wave_max(m);
+}
+
+/* CFL of one block and stage from its slots (Main.c:395), the production
+ * part refolded after a NaN; clears the slots. All 64 lanes call; the
+ * result is uniform. */
+template <int STAGE>
+__device__ double block_cfl(unsigned long long *s, const BlockDev &B, const double *in, double dt,
+                            double dtkn, double force)
+{
+  const double cx = __longlong_as_double((long long)s[0]);
+  const double cy = __longlong_as_double((long long)s[1]);
+  double cprod = __longlong_as_double((long long)s[2]);
+  const unsigned long long pnan = s[3];
+  if (pnan) cprod = prod_after_nan<STAGE>(in, B, pnan, dt, dtkn, force);
+  __syncthreads();
+  if (threadIdx.x == 0) { s[0] = 0ull; s[1] = 0ull; s[2] = 0ull; s[3] = 0ull; }
+  return dt * cx / B.dx + dt * cy / B.dy + cprod;
+}
+
+/* Steps controller of every deck (Main.c:202-216) after one TimeUpdate:
+ * combined CFL (Main.c:395, :268-282; a block whose CFL is NaN is skipped,
+ * as the reference's `cfl > CFL1` skips it), accept/reject, dt update, clamp
+ * to tend, parity flip instead of the New->Data copy; clears the slots.
+ * One 64-lane workgroup per deck. */
+__global__ void __launch_bounds__(64) control_kernel(DeckState *decks, unsigned long long *slots,
+                                                     const BlockDev *blocks, const int *first, int nb,
+                                                     RecordDev *rec, const double *buf0,
+                                                     const double *buf1, const double *tmp)
+{
+  const int d = blockIdx.x;
   DeckState S = decks[d];
   if (S.done) return;
+  const double dtkn = S.dt / S.kn;
+  const double *data = S.parity ? buf1 : buf0;
   double c1 = 0.0, c2 = 0.0;
-  for (int st = 0; st < 2; st++)
-    for (int k = first[d]; k < first[d + 1]; k++) {
-      unsigned long long *s = slots + (size_t)3 * (st * nb + k);
-      const double cx = __longlong_as_double((long long)s[0]);
-      const double cy = __longlong_as_double((long long)s[1]);
-      const double cp = __longlong_as_double((long long)s[2]);
-      const double v = S.dt * cx / blocks[k].dx + S.dt * cy / blocks[k].dy + cp;
-      if (st == 0) { if (v > c1) c1 = v; }
-      else { if (v > c2) c2 = v; }
-      s[0] = 0ull; s[1] = 0ull; s[2] = 0ull;
-    }
+  for (int k = first[d]; k < first[d + 1]; k++) {
+    const double v = block_cfl<1>(slots + (size_t)SLOTS_PER_BLOCK * k, blocks[k], data, S.dt, dtkn, S.force);
+    if (v > c1) c1 = v;
+  }
+  for (int k = first[d]; k < first[d + 1]; k++) {
+    const double v = block_cfl<2>(slots + (size_t)SLOTS_PER_BLOCK * (nb + k), blocks[k], tmp, S.dt,
+                                  dtkn, S.force);
+    if (v > c2) c2 = v;
+  }
+  if (threadIdx.x != 0) return;
   const double m2 = c2 > 0.0 ? c2 : 0.0;
   const double cfl = c1 > m2 ? c1 : m2;
   const double used = S.dt;
@@ -522,23 +706,23 @@ __global__ void control_kernel(DeckState *decks, unsigned long long *slots,
 }
 
 /* Combined CFL (Main.c:395 per block, Main.c:268-282 over blocks and stages),
- * evaluated in the reference's order; clears the slots for the next step. */
-__global__ void finalize_kernel(unsigned long long *slots, const BlockDev *blocks, int nb,
-                                double dt, double *out)
+ * evaluated in the reference's order; clears the slots for the next step.
+ * One 64-lane workgroup; in1/in2: the inputs of stages 1 and 2. */
+__global__ void __launch_bounds__(64) finalize_kernel(unsigned long long *slots, const BlockDev *blocks,
+                                                      int nb, double dt, double kn, double force,
+                                                      const double *in1, const double *in2, double *out)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double c1 = 0.0, c2 = 0.0;
-  for (int st = 0; st < 2; st++)
-    for (int k = 0; k < nb; k++) {
-      unsigned long long *s = slots + (size_t)3 * (st * nb + k);
-      const double cx = __longlong_as_double((long long)s[0]);
-      const double cy = __longlong_as_double((long long)s[1]);
-      const double cp = __longlong_as_double((long long)s[2]);
-      const double v = dt * cx / blocks[k].dx + dt * cy / blocks[k].dy + cp;
-      if (st == 0) { if (v > c1) c1 = v; }
-      else { if (v > c2) c2 = v; }
-      s[0] = 0ull; s[1] = 0ull; s[2] = 0ull;
-    }
+  for (int k = 0; k < nb; k++) {
+    const double v = block_cfl<1>(slots + (size_t)SLOTS_PER_BLOCK * k, blocks[k], in1, dt, dt / kn, force);
+    if (v > c1) c1 = v;
+  }
+  for (int k = 0; k < nb; k++) {
+    const double v = block_cfl<2>(slots + (size_t)SLOTS_PER_BLOCK * (nb + k), blocks[k], in2, dt, dt / kn,
+                                  force);
+    if (v > c2) c2 = v;
+  }
+  if (threadIdx.x != 0) return;
   const double m2 = c2 > 0.0 ? c2 : 0.0;
   out[0] = c1 > m2 ? c1 : m2;
 }
@@ -1132,8 +1316,8 @@ mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_f
       (fr.empty() || hipok(hipMemcpy(c->dfill, fr.data(), fr.size() * sizeof(FillRec), hipMemcpyHostToDevice), "hipMemcpy")) &&
       hipok(hipMalloc(&c->dfillxy, std::max<size_t>(1, fxy.size()) * sizeof(double2)), "hipMalloc") &&
       (fxy.empty() || hipok(hipMemcpy(c->dfillxy, fxy.data(), fxy.size() * sizeof(double2), hipMemcpyHostToDevice), "hipMemcpy")) &&
-      hipok(hipMalloc(&c->dslots, (size_t)6 * nblocks * sizeof(unsigned long long)), "hipMalloc") &&
-      hipok(hipMemsetAsync(c->dslots, 0, (size_t)6 * nblocks * sizeof(unsigned long long), c->stream), "hipMemsetAsync") &&
+      hipok(hipMalloc(&c->dslots, (size_t)2 * SLOTS_PER_BLOCK * nblocks * sizeof(unsigned long long)), "hipMalloc") &&
+      hipok(hipMemsetAsync(c->dslots, 0, (size_t)2 * SLOTS_PER_BLOCK * nblocks * sizeof(unsigned long long), c->stream), "hipMemsetAsync") &&
       hipok(hipMalloc(&c->dcfl, sizeof(double)), "hipMalloc") &&
       hipok(hipMalloc(&c->dtj, 5 * sizeof(double)), "hipMalloc") &&
       hipok(hipHostMalloc(&c->hcfl, sizeof(double)), "hipHostMalloc") &&
@@ -1178,7 +1362,7 @@ int enqueue_group_stage(mbm_ctx *c, int g, int stage, double *buf0, double *buf1
   A.buf0 = buf0;
   A.buf1 = buf1;
   A.tmp = tmp;
-  A.slots = c->dslots + (stage == 1 ? 0 : 3 * c->nb);
+  A.slots = c->dslots + (stage == 1 ? 0 : SLOTS_PER_BLOCK * c->nb);
   int rc;
   if ((rc = stage == 1 ? launch_stage<1>(c, A) : launch_stage<2>(c, A))) return rc;
   if (h1 > h0)
@@ -1189,12 +1373,12 @@ int enqueue_group_stage(mbm_ctx *c, int g, int stage, double *buf0, double *buf1
   return MBM_OK;
 }
 
-int enqueue_group_control(mbm_ctx *c, int g)
+int enqueue_group_control(mbm_ctx *c, int g, const double *buf0, const double *buf1, const double *tmp)
 {
   const int d0 = g < 0 ? 0 : c->grp_deck[g], d1 = g < 0 ? c->ndecks : c->grp_deck[g + 1];
-  hipLaunchKernelGGL(control_kernel, dim3((d1 - d0 + 63) / 64), dim3(64), 0, c->stream,
-                     c->ddecks + d0, c->dslots, c->dblk, c->ddeck_first + d0, d1 - d0, c->nb,
-                     g > 0 ? (RecordDev *)nullptr : c->drec);
+  hipLaunchKernelGGL(control_kernel, dim3(d1 - d0), dim3(64), 0, c->stream,
+                     c->ddecks + d0, c->dslots, c->dblk, c->ddeck_first + d0, c->nb,
+                     g > 0 ? (RecordDev *)nullptr : c->drec, buf0, buf1, tmp);
   HIPCHK(hipGetLastError());
   return MBM_OK;
 }
@@ -1205,7 +1389,7 @@ int enqueue_device_step(mbm_ctx *c, double *buf0, double *buf1, double *tmp)
   int rc;
   if ((rc = enqueue_group_stage(c, -1, 1, buf0, buf1, tmp))) return rc;
   if ((rc = enqueue_group_stage(c, -1, 2, buf0, buf1, tmp))) return rc;
-  return enqueue_group_control(c, -1);
+  return enqueue_group_control(c, -1, buf0, buf1, tmp);
 }
 
 } // namespace
@@ -1225,13 +1409,13 @@ int enqueue_device_stages(mbm_ctx *c)
   A.slots = c->dslots;
   int rc;
   if ((rc = launch_stage<1>(c, A))) return rc;
-  A.slots = c->dslots + 3 * c->nb;
+  A.slots = c->dslots + SLOTS_PER_BLOCK * c->nb;
   return launch_stage<2>(c, A);
 }
 
 int clear_slots(mbm_ctx *c)
 {
-  const int n = 6 * c->nb;
+  const int n = 2 * SLOTS_PER_BLOCK * c->nb;
   hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->dslots, n);
   HIPCHK(hipGetLastError());
   return MBM_OK;
@@ -1275,7 +1459,7 @@ int step_graph(mbm_ctx *c, int iD, int iN, int iT)
     rc = capture(c, c->stream, &c->graph[iD][iN], [&] {
       int r = enqueue_group_stage(c, 0, 1, buf0, buf1, tmp);
       if (!r) r = enqueue_group_stage(c, 0, 2, buf0, buf1, tmp);
-      return r ? r : enqueue_group_control(c, 0);
+      return r ? r : enqueue_group_control(c, 0, buf0, buf1, tmp);
     });
   else
     rc = capture(c, c->stream, &c->graph[iD][iN],
@@ -1300,7 +1484,7 @@ int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known)
     /* group 1, half an attempt later: stage 2, controller, next stage 1 */
     rc = capture(c, c->stream2, &c->graph2, [&] {
       int r = enqueue_group_stage(c, 1, 2, buf0, buf1, tmp);
-      if (!r) r = enqueue_group_control(c, 1);
+      if (!r) r = enqueue_group_control(c, 1, buf0, buf1, tmp);
       return r ? r : enqueue_group_stage(c, 1, 1, buf0, buf1, tmp);
     });
     if (rc) return rc;
@@ -1561,13 +1745,14 @@ int enqueue_time_update(mbm_ctx *c, double dt)
   A.in = c->layer[c->role[MBM_L_TMP]];
   A.in0 = c->layer[c->role[MBM_L_DATA]];
   A.out = c->layer[c->role[MBM_L_NEW]];
-  A.slots = c->dslots + 3 * c->nb;
+  A.slots = c->dslots + SLOTS_PER_BLOCK * c->nb;
   if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->stream));
   if ((rc = launch_stage<2>(c, A))) return rc;
   if (c->timing) HIPCHK(hipEventRecord(c->ev[3], c->stream));
   if ((rc = do_fill(c, MBM_L_NEW))) return rc;
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, c->stream, c->dslots, c->dblk, c->nb,
-                     dt, c->dcfl);
+                     dt, c->par.kn, c->par.force, c->layer[c->role[MBM_L_DATA]],
+                     c->layer[c->role[MBM_L_TMP]], c->dcfl);
   HIPCHK(hipGetLastError());
   return MBM_OK;
 }

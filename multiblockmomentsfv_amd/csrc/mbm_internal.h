@@ -30,6 +30,10 @@ int set_err(int code, const char *fmt, ...);
 constexpr int NV = 9;
 constexpr int MBC = 3;
 constexpr int STRIP = 60;          /* owned columns per wavefront */
+/* CFL slots per block and stage: max x-face speed, max y-face speed, max
+ * production CFL (bit patterns of non-negative doubles, NaN ignored), and 1 +
+ * the scan position of the last NaN production CFL (0: none) */
+constexpr int SLOTS_PER_BLOCK = 4;
 constexpr int WAVES_PER_WG = 4;
 
 /* A block in a layer buffer. Rows are stored at row positions: a position
@@ -125,7 +129,7 @@ struct StageArgs {
   double *out;        /* stage 1: TMP, stage 2: NEW */
   const BlockDev *blocks;
   const TileDev *tiles;
-  unsigned long long *slots;   /* per block: max x-face speed, y-face speed, prod CFL */
+  unsigned long long *slots;   /* SLOTS_PER_BLOCK per block */
   int ntiles;
   double dt, dtkn, force;
   /* device-controlled mode (decks != nullptr): dt, Kn and F come from the
@@ -162,7 +166,7 @@ struct mbm_ctx {
   FillRec *dfill;             /* ghost fill records, in block order */
   double2 *dfillxy;           /* ghost cell centres (scenario ghosts) */
   int nghost;
-  unsigned long long *dslots; /* 2 stages x nb x 3 */
+  unsigned long long *dslots; /* 2 stages x nb x SLOTS_PER_BLOCK */
   double *dcfl;               /* combined CFL */
   double *dtj;                /* scenario params */
   double *hcfl;               /* pinned */

@@ -15,7 +15,18 @@
  *  - the slope division by dx (Main.c:346-349) and the 0.5*dx factor of the
  *    reconstruction (Main.c:360-374) cancel and are dropped.
  * Results agree with the reference to rounding (<= 1e-12 relative L-inf per
- * field on the checked decks); NaN propagation is not reproduced.
+ * field on the checked decks).
+ *
+ * Non-finite data: these fast forms give the reference's NaN pattern only on
+ * finite data with a nonzero face density. Where a NaN is present, the
+ * reference's exact expression structure decides which components turn NaN
+ * (its limiters compare against NaN, sgn(NaN) = 1 and min(NaN, b) = b, its
+ * flux divides by the density after Conservative() and rotates components
+ * into the face frame, where NaN * 0 is NaN). The *_ref functions at the end
+ * restate those expressions; the stage kernel switches a row to them when a
+ * primitive in the row's stencil is not finite. A face whose reference
+ * density is exactly zero has an all-NaN flux (0/0 in Flux.c:85-103); the
+ * kernel decides that exact zero in the reference's arithmetic.
  */
 #ifndef MBM_PHYS_H
 #define MBM_PHYS_H
@@ -67,6 +78,17 @@ MBM_DEV double sqrtp(double x)
 #else
   return sqrt(x);
 #endif
+}
+
+/* true when the relaxed components 4..8 of a projection are finite (their
+ * sum is; an overflowing sum of finite values only errs towards the slower
+ * reference-form path). A non-finite conservative component reaches them:
+ * U[0..3] through theta and the damping factors, U[4..8] directly; so does
+ * a NaN damping factor (negative temperature). */
+MBM_DEV bool finite_relaxed(const double *P)
+{
+  const double s = (P[4] + P[5]) + (P[6] + P[7]) + P[8];
+  return __builtin_isfinite(s);
 }
 
 /* primitive variables {rho, v1, v2, theta, T11, T12, T22, q1, q2} */
@@ -327,15 +349,25 @@ MBM_DEV void face(const double *L, const double *R, double dtdx, double *Fs,
   for (int n = 0; n < 9; n++) Fs[perm<DIR>(n)] = Ff[n];
 }
 
+/* a face whose reference density is exactly zero: every component of the
+ * reference's flux is 0/0 (Flux.c:85-103 divide by rho after Conservative) */
+MBM_DEV void nan_face(double *Fs)
+{
+#pragma unroll
+  for (int n = 0; n < 9; n++) Fs[n] = __builtin_nan("");
+}
+
 /* Projection of one loaded cell (Main.c:314-336):
  *   P  = primitives used for slopes and faces: relaxed with flag 1 (stage 1)
  *        or 3 (stage 2) on rings <= 2, unrelaxed on ring 3;
  *   I  = initial value of the new state: relaxation flag 0 (stage 1) or
  *        flag 2 of the stage input plus flag -1 of DATA (stage 2);
  *   cp = production CFL 0.5*(1-R0(2z/3)) (stage 1 only). */
+/* bad is set when the relaxed primitives are not all finite (a non-finite
+ * input, or a NaN damping factor from a negative temperature) */
 template <int STAGE, bool NEED_INIT>
 MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
-                     double dtkn, double force, double *P, double *I, double &cp)
+                     double dtkn, double force, double *P, double *I, double &cp, bool &bad)
 {
   double Q[9];
   primitive(U, Q);
@@ -344,6 +376,7 @@ MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
   damp_inv(z, i1, i2);
   if constexpr (STAGE == 1) {
     relax(Q, damp1(z, i1, i2), P);
+    bad = !finite_relaxed(P);
     if constexpr (NEED_INIT) {
       const Damp d0 = damp0(z, i1, i2);
       double R[9];
@@ -357,6 +390,7 @@ MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
     d3.d1 = i1;
     d3.d2 = i2;
     relax(Q, d3, P);
+    bad = !finite_relaxed(P);
     if constexpr (NEED_INIT) {
       double R[9], Q0[9], V[9];
       if (force != 0.0) {
@@ -365,10 +399,19 @@ MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
         for (int n = 0; n < 9; n++) R[n] = Q[n];
         R[1] += dt * force;
         conservative(R, I);
-      } else {
+      } else if (__builtin_expect(!bad, 1)) {
         /* without force flag 2 is the identity: Conservative(Primitive(U)) = U */
 #pragma unroll
         for (int n = 0; n < 9; n++) I[n] = U[n];
+      } else {
+        /* non-finite data: the round trip of Flux.c:230-267 with the factor
+         * R2(z) = (1+z/2)/(1+z/2) (Flux.c:220), which is NaN when z is */
+        const double a = 1.0 + 0.5 * z, two = 1.0 + 0.5 * ((2.0 / 3.0) * z);
+        Damp dI;
+        dI.d1 = a / a;
+        dI.d2 = two / two;
+        relax(Q, dI, R);
+        conservative(R, I);
       }
       primitive(U0, Q0);
       const double z0 = Q0[0] * sqrtp(Q0[3]) * dtkn;
@@ -378,12 +421,237 @@ MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
       conservative(R, V);
 #pragma unroll
       for (int n = 0; n < 9; n++) I[n] += V[n];
-      cp = 0.0;
+      /* production CFL of flag 2 (Flux.c:264): 0.5*(1 - R2(2z/3)), zero unless
+       * that factor is 0/0 or inf/inf */
+      const double two = 1.0 + 0.5 * ((2.0 / 3.0) * z);
+      cp = (__builtin_isfinite(two) && two != 0.0) ? 0.0 : __builtin_nan("");
     }
   }
   if (ring3) {
 #pragma unroll
     for (int n = 0; n < 9; n++) P[n] = Q[n];
+  }
+}
+
+template <int STAGE, bool NEED_INIT>
+MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
+                     double dtkn, double force, double *P, double *I, double &cp)
+{
+  bool bad;
+  project<STAGE, NEED_INIT>(U, U0, ring3, dt, dtkn, force, P, I, cp, bad);
+}
+
+/* ---------------- reference-form paths for non-finite data ----------------
+ * Restatements of the reference expressions whose NaN behaviour the fast
+ * forms do not share; used only for rows whose stencil holds a non-finite
+ * primitive. Values agree with the fast forms to rounding. */
+
+MBM_DEV double ref_max(double a, double b) { return a > b ? a : b; }          /* tools.h:3-6 */
+MBM_DEV double ref_min(double a, double b) { return a < b ? a : b; }          /* tools.h:8-11 */
+MBM_DEV double ref_sgn(double a) { return a == 0 ? 0.0 : (a < 0 ? -1.0 : 1.0); }   /* tools.h:13-17 */
+
+/* Limit1 (Limiter.c:26-59) without the /dx that the reconstruction cancels */
+template <int LIM>
+MBM_DEV double limit_ref(double d1, double d2)
+{
+#pragma clang fp contract(off)
+  if constexpr (LIM == 0) {
+    return 0.0;
+  } else if constexpr (LIM == 1) {
+    return 0.5 * (d1 + d2);
+  } else if constexpr (LIM == 2) {
+    return 0.5 * (ref_sgn(d1) + ref_sgn(d2)) * ref_min(fabs(d1), fabs(d2));
+  } else if constexpr (LIM == 3) {
+    const double r = d2 / d1;
+    double phi = 2.0;
+    if (r < 0.0) phi = 0.0;
+    else if (r < 0.5) phi = 2.0 * r;
+    else if (r < 1.0) phi = 1.0;
+    else if (r < 2.0) phi = r;
+    return d1 * phi;
+  } else {
+    const double w1 = fabs(d1) + 1.0e-10, w2 = fabs(d2) + 1.0e-10;
+    return (w1 * d2 + w2 * d1) / (w1 + w2);
+  }
+}
+
+/* Face states of one cell as Main.c:346-374 forms them, from the one-sided
+ * differences d1 = p - p_minus, d2 = p_plus - p: slope Limit1(d1, d2)/dx,
+ * then p -/+ (0.5*dx)*slope for the lower/upper face, each operation
+ * rounded on its own as in the reference build */
+template <int LIM>
+MBM_DEV void reconstruct_ref(double p, double d1, double d2, double dx, double &lo, double &hi)
+{
+#pragma clang fp contract(off)
+  const double sl = limit_ref<LIM>(d1, d2) / dx;
+  const double h = 0.5 * dx;
+  lo = p - h * sl;
+  hi = p + h * sl;
+}
+
+/* The reference's face densities of a cell may be exactly zero only where
+ * the fast ones, flo = p - s/2 and fhi = p + s/2 (s the fast limited slope),
+ * are within 1e-13 |p| of zero: both forms agree to a few ulps of p. */
+MBM_DEV bool maybe_zero_density(double p, double flo, double fhi)
+{
+  const double tiny = 1e-13 * fabs(p);
+  return fabs(flo) <= tiny || fabs(fhi) <= tiny;
+}
+
+/* Whether the reference's face densities of a cell are exactly zero, decided
+ * in the reference's arithmetic where maybe_zero_density holds (the density
+ * is never transformed by the projection, so p and the differences are the
+ * reference's own values). */
+template <int LIM>
+MBM_DEV void zero_density_ref(double p, double d1, double d2, double dx, double flo, double fhi,
+                              bool &zlo, bool &zhi)
+{
+  zlo = zhi = false;
+  if (__builtin_expect(maybe_zero_density(p, flo, fhi), 0)) {
+    double lo, hi;
+    reconstruct_ref<LIM>(p, d1, d2, dx, lo, hi);
+    zlo = lo == 0.0;
+    zhi = hi == 0.0;
+  }
+}
+
+/* Conservative with E from primDef.h (Flux.c:61-74) */
+MBM_DEV void conservative_ref(const double *P, double *U)
+{
+#pragma clang fp contract(off)
+  const double rho = P[0], v1 = P[1], v2 = P[2], th = P[3];
+  const double vs = v1 * v1 + v2 * v2;
+  const double E = 0.5 * rho * vs + 1.5 * rho * th;
+  U[0] = rho;
+  U[1] = rho * v1;
+  U[2] = rho * v2;
+  U[3] = E;
+  U[4] = rho * (P[4] + v1 * v1);
+  U[5] = rho * (P[5] + v1 * v2);
+  U[6] = rho * (P[6] + v2 * v2);
+  U[7] = P[7] + E * v1 + rho * (P[4] * v1 + P[5] * v2);
+  U[8] = P[8] + E * v2 + rho * (P[5] * v1 + P[6] * v2);
+}
+
+/* component n of conservative_ref, with the same expressions (so the same
+ * rounding) */
+MBM_DEV double conservative_ref_comp(const double *P, int n)
+{
+#pragma clang fp contract(off)
+  const double rho = P[0], v1 = P[1], v2 = P[2], th = P[3];
+  const double vs = v1 * v1 + v2 * v2;
+  const double E = 0.5 * rho * vs + 1.5 * rho * th;
+  switch (n) {
+    case 0: return rho;
+    case 1: return rho * v1;
+    case 2: return rho * v2;
+    case 3: return E;
+    case 4: return rho * (P[4] + v1 * v1);
+    case 5: return rho * (P[5] + v1 * v2);
+    case 6: return rho * (P[6] + v2 * v2);
+    case 7: return P[7] + E * v1 + rho * (P[4] * v1 + P[5] * v2);
+    default: return P[8] + E * v2 + rho * (P[5] * v1 + P[6] * v2);
+  }
+}
+
+/* Grad13Flux1D (Flux.c:79-119): primitives recovered by division */
+MBM_DEV void flux1d_ref(const double *U, double *F)
+{
+#pragma clang fp contract(off)
+  const double rho = U[0], v1 = U[1] / rho, v2 = U[2] / rho;
+  const double vs = v1 * v1 + v2 * v2;
+  const double th = (2 * U[3] / rho - vs) / 3, E = U[3];
+  const double T11 = U[4] / rho - v1 * v1, T12 = U[5] / rho - v1 * v2, T22 = U[6] / rho - v2 * v2;
+  const double q1 = U[7] - E * v1 - rho * (T11 * v1 + T12 * v2);
+  const double q2 = U[8] - E * v2 - rho * (T12 * v1 + T22 * v2);
+  const double M111 = 1.2 * q1 / rho, M121 = 0.4 * q2 / rho, M112 = M121;
+  const double M221 = 0.4 * q1 / rho, M122 = M221;
+  const double R11 = 5 * th * th + 7 * th * (T11 - th), R12 = 7 * th * T12;
+  F[0] = rho * v1;
+  F[1] = rho * (T11 + v1 * v1);
+  F[2] = rho * (T12 + v1 * v2);
+  F[3] = E * v1 + rho * (T11 * v1 + T12 * v2) + q1;
+  F[4] = rho * (M111 + 3 * T11 * v1 + v1 * v1 * v1);
+  F[5] = rho * (M121 + T11 * v2 + 2 * T12 * v1 + v1 * v2 * v1);
+  F[6] = rho * (M221 + T22 * v1 + 2 * T12 * v2 + v2 * v2 * v1);
+  F[7] = 2 * (q1 + rho * (T11 * v1 + T12 * v2)) * v1 + E * v1 * v1 + rho * (M111 * v1 + M112 * v2)
+       + 0.5 * rho * (T11 * vs + R11);
+  F[8] = q2 * v1 + q1 * v2 + rho * (v1 * v2 * (T11 + T22) + 1.5 * T12 * vs) + E * v1 * v2
+       + rho * (M121 * v1 + M122 * v2) + 0.5 * rho * R12;
+}
+
+/* Grad13Flux2D (Flux.c:126-172) for n = (nx, ny), t = (ny, -nx); the
+ * products with 0 and 1 are kept: they carry NaN across components */
+MBM_DEV void flux2d_ref(const double *U, double *F, double nx, double ny)
+{
+#pragma clang fp contract(off)
+  const double tx = ny, ty = -nx;
+  double UR[9], FR[9], s1, s2;
+  UR[0] = U[0];
+  UR[1] = U[1] * nx + U[2] * ny;
+  UR[2] = U[1] * tx + U[2] * ty;
+  UR[3] = U[3];
+  s1 = U[4] * nx + U[5] * ny;
+  s2 = U[5] * nx + U[6] * ny;
+  UR[4] = s1 * nx + s2 * ny;
+  UR[5] = s1 * tx + s2 * ty;
+  s1 = U[4] * tx + U[5] * ty;
+  s2 = U[5] * tx + U[6] * ty;
+  UR[6] = s1 * tx + s2 * ty;
+  UR[7] = U[7] * nx + U[8] * ny;
+  UR[8] = U[7] * tx + U[8] * ty;
+  flux1d_ref(UR, FR);
+  F[0] = FR[0];
+  F[1] = FR[1] * nx + FR[2] * tx;
+  F[2] = FR[1] * ny + FR[2] * ty;
+  F[3] = FR[3];
+  s1 = FR[4] * nx + FR[5] * tx;
+  s2 = FR[5] * nx + FR[6] * tx;
+  F[4] = s1 * nx + s2 * tx;
+  F[5] = s1 * ny + s2 * ty;
+  s1 = FR[4] * ny + FR[5] * ty;
+  s2 = FR[5] * ny + FR[6] * ty;
+  F[6] = s1 * ny + s2 * ty;
+  F[7] = FR[7] * nx + FR[8] * tx;
+  F[8] = FR[7] * ny + FR[8] * ty;
+}
+
+/* FnumHLLE (Flux.c:332-379) on Cartesian primitives L, R, x-face (DIR 0) or
+ * y-face (DIR 1); Fs = (dt/dx) * flux, smax = cMax (lamDiff is 1 there).
+ * The combine keeps the reference's order, lamR fL - lamL fR + lamL lamR
+ * (UR - UL), so that values agree to rounding also where UR - UL cancels;
+ * UL and UR are formed per component when they are needed, which keeps few
+ * temporaries live (the stage kernel runs this inside its march). */
+template <int DIR>
+MBM_DEV void face_ref(const double *L, const double *R, double dtdx, double *Fs, double &smax)
+{
+#pragma clang fp contract(off)
+  const double nx = DIR == 0 ? 1.0 : 0.0, ny = DIR == 0 ? 0.0 : 1.0;
+  const double cR = 2.0 * sqrt(5.0 / 3 * R[3]), cL = 2.0 * sqrt(5.0 / 3 * L[3]);
+  const double vL = L[1 + DIR], vR = R[1 + DIR];
+  double fL[9], fR[9];
+  {
+    double U[9];
+    conservative_ref(L, U);
+    flux2d_ref(U, fL, nx, ny);
+  }
+  {
+    double U[9];
+    conservative_ref(R, U);
+    flux2d_ref(U, fR, nx, ny);
+  }
+  smax = fabs(0.5 * (vL + vR)) + ref_max(cR, cL);
+  const double lamL = ref_min(vL - cL, 0.0), lamR = ref_max(vR + cR, 0.0);
+  const double alph = lamR - lamL;
+  if (alph > 0.0) {
+#pragma unroll
+    for (int n = 0; n < 9; n++) {
+      const double uL = conservative_ref_comp(L, n), uR = conservative_ref_comp(R, n);
+      Fs[n] = dtdx * ((lamR * fL[n] - lamL * fR[n] + lamL * lamR * (uR - uL)) / alph);
+    }
+  } else {
+#pragma unroll
+    for (int n = 0; n < 9; n++) Fs[n] = dtdx * (0.5 * (fL[n] + fR[n]));
   }
 }
 

@@ -47,15 +47,35 @@ def interior(a):
     return a[3:-3, 3:-3, :]
 
 
-def rel_linf(a_list, b_list, cells=None):
-    """Per-field relative L-inf difference over all blocks."""
+def field_scale(b_list):
+    """Per-field max |value| over the finite values of a list of blocks."""
+    return np.maximum.reduce([np.where(np.isfinite(b), np.abs(b), 0.0).reshape(-1, 9).max(axis=0)
+                              for b in b_list])
+
+
+def rel_linf(a_list, b_list, cells=None, scale=None):
+    """Per-field relative L-inf difference over all blocks. Non-finite values
+    must sit at the same places in both (NaN where the reference has NaN,
+    +-inf with the same sign), else every field reports inf; the difference
+    is then taken over the finite values, relative to max |b| per field or
+    to the given per-field scale (for states that are mostly NaN, whose
+    finite remainder can be cancellation residue far below the field's
+    size)."""
     num = np.zeros(9)
     den = np.zeros(9)
     for a, b in zip(a_list, b_list):
         if cells is not None:
             a, b = cells(a), cells(b)
-        num = np.maximum(num, np.abs(a - b).reshape(-1, 9).max(axis=0))
-        den = np.maximum(den, np.abs(b).reshape(-1, 9).max(axis=0))
+        a = a.reshape(-1, 9)
+        b = b.reshape(-1, 9)
+        fa, fb = np.isfinite(a), np.isfinite(b)
+        if not (np.array_equal(fa, fb) and np.array_equal(np.isnan(a), np.isnan(b))
+                and np.array_equal(a[~fa & ~np.isnan(a)], b[~fb & ~np.isnan(b)])):
+            return np.full(9, np.inf)
+        num = np.maximum(num, np.where(fb, np.abs(a - b), 0.0).max(axis=0))
+        den = np.maximum(den, np.where(fb, np.abs(b), 0.0).max(axis=0))
+    if scale is not None:
+        den = np.asarray(scale, dtype=np.float64)
     return num / np.where(den > 0, den, 1.0)
 
 
