@@ -1,0 +1,78 @@
+"""Full-length runs at the configured sizes against the reference build.
+
+tests/golden/long/<name>.npz (tests/golden/make_long_fixtures.py) keeps, of a
+reference run of the Steps controller (Main.c:189-244) on the full-size deck,
+every attempt's CFL, dt and time, per-block/per-field statistics of the final
+state and FP64 rows and columns through the initial jump:
+    square4_1024  4 x 1024^2, Reg13, Kn 0.01, CFL 0.9, harmonic, to t = 0.3
+    lshape_500    L-shape 2048x1024 / 1024^2 / 1024x2048, minmod, Kn 0.05,
+                  the first 500 steps of the run to t = 0.6
+The GPU runs the same steps. Bar: the same number of attempts and the same
+accept/reject sequence; dt, CFL and t of every attempt, the statistics and
+the lines within LONG_TOL relative (per field: max |a - b| / max |b|).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+LONG = os.path.join(ROOT, "tests", "golden", "long")
+NAMES = sorted(f[:-4] for f in os.listdir(LONG) if f.endswith(".npz")) if os.path.isdir(LONG) else []
+LONG_TOL = 1e-12
+
+
+def rel(a, b):
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_full_length_run_matches_the_reference(mbm, name):
+    z = np.load(os.path.join(LONG, name + ".npz"), allow_pickle=False)
+    log = z["log"]
+    deck = mbm.read_deck(os.path.join(ROOT, str(z["deck"])))
+    eng = mbm.Engine.from_deck(deck)
+    eng.init()
+    nacc = int(log[:, 3].sum())
+    # a prefix of the run (the reference stopped before the output time)
+    # stops after the same number of accepted steps
+    prefix = log[-1, 2] < deck.tout[0]
+    t, n, nrej, mine = eng.steps(0.0, deck.tout[0], deck.cfl, maxsteps=nacc if prefix else 0,
+                                 reccap=len(log) + 16)
+    print(f"\n{name}: {n} steps, {nrej} rejections, t = {t!r}")
+    assert n == nacc and len(mine) == len(log)
+    assert np.array_equal(mine[:, 3], log[:, 3]), "accept/reject sequence differs"
+    drift = {k: rel(mine[:, c], log[:, c]) for k, c in (("cfl", 0), ("dt", 1), ("t", 2))}
+    # the drift along the run, in quarters
+    q = [rel(mine[i:i + len(log) // 4, 0], log[i:i + len(log) // 4, 0])
+         for i in range(0, len(log) - len(log) // 4 + 1, max(1, len(log) // 4))]
+    print(f"  attempts: rel diff cfl {drift['cfl']:.2e} dt {drift['dt']:.2e} t {drift['t']:.2e}; "
+          f"cfl by quarter {['%.1e' % v for v in q]}")
+    blocks = eng.download(0)
+    stats = []
+    for b in blocks:
+        inner = b[3:-3, 3:-3, :].reshape(-1, 9)
+        stats.append(np.stack([inner.max(axis=0), inner.min(axis=0), np.abs(inner).sum(axis=0),
+                               np.sqrt((inner * inner).sum(axis=0))], axis=1))
+    stats = np.array(stats)
+    ref = z["stats"]
+    s_err = max(rel(stats[:, m, j], ref[:, m, j]) for m in range(9) for j in range(4))
+    # lines: per field, relative to the field's size over the whole state
+    # (the parity measure); a field can be ~0 along one line (rho v_y on y = 0)
+    size = np.maximum(np.abs(ref[:, :, 0]), np.abs(ref[:, :, 1])).max(axis=0)
+    l_err = 0.0
+    per_field = np.zeros(9)
+    for n_, (k, kind, idx) in enumerate(z["line_index"]):
+        got = blocks[k][3 + idx, 3:-3, :] if kind == 0 else blocks[k][3:-3, 3 + idx, :]
+        want = z[f"line{n_}"]
+        per_field = np.maximum(per_field, np.abs(got - want).max(axis=0) / size)
+    l_err = float(per_field.max())
+    print("  lines per field:", " ".join("%.1e" % v for v in per_field))
+    print(f"  final state: statistics {s_err:.2e}, rows/columns through the jump {l_err:.2e}")
+    eng.close()
+    assert max(drift.values()) <= LONG_TOL
+    assert s_err <= LONG_TOL and l_err <= LONG_TOL
