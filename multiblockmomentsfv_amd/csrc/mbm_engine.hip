@@ -1335,14 +1335,23 @@ mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_f
 void release_graphs(mbm_ctx *c)
 {
   for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++)
+    for (int j = 0; j < 3; j++) {
       if (c->graph_ok[i][j]) {
         (void)hipGraphExecDestroy(c->graph[i][j]);
         c->graph_ok[i][j] = false;
       }
+      if (c->graphk_ok[i][j]) {
+        (void)hipGraphExecDestroy(c->graphk[i][j]);
+        c->graphk_ok[i][j] = false;
+      }
+    }
   if (c->graph2_ok) {
     (void)hipGraphExecDestroy(c->graph2);
     c->graph2_ok = false;
+  }
+  if (c->graph2k_ok) {
+    (void)hipGraphExecDestroy(c->graph2k);
+    c->graph2k_ok = false;
   }
 }
 
@@ -1395,22 +1404,15 @@ int enqueue_device_step(mbm_ctx *c, double *buf0, double *buf1, double *tmp)
 } // namespace
 
 /* stages and fills of every deck at its current dt, without the controller
- * (writes TMP and NEW only; used to time tile shapes) */
+ * (writes TMP and NEW only; used to time tile shapes). The fills belong to
+ * the timed work, and without them stage 2 would read stale TMP ghosts. */
 int enqueue_device_stages(mbm_ctx *c)
 {
-  StageArgs A{};
-  A.blocks = c->dblk;
-  A.tiles = c->dtiles;
-  A.ntiles = c->ntiles;
-  A.decks = c->ddecks;
-  A.buf0 = c->layer[c->role[MBM_L_DATA]];
-  A.buf1 = c->layer[c->role[MBM_L_NEW]];
-  A.tmp = c->layer[c->role[MBM_L_TMP]];
-  A.slots = c->dslots;
+  double *buf0 = c->layer[c->role[MBM_L_DATA]], *buf1 = c->layer[c->role[MBM_L_NEW]];
+  double *tmp = c->layer[c->role[MBM_L_TMP]];
   int rc;
-  if ((rc = launch_stage<1>(c, A))) return rc;
-  A.slots = c->dslots + SLOTS_PER_BLOCK * c->nb;
-  return launch_stage<2>(c, A);
+  if ((rc = enqueue_group_stage(c, -1, 1, buf0, buf1, tmp))) return rc;
+  return enqueue_group_stage(c, -1, 2, buf0, buf1, tmp);
 }
 
 int clear_slots(mbm_ctx *c)
@@ -1449,23 +1451,56 @@ int capture(mbm_ctx *c, hipStream_t s, hipGraphExec_t *out, F enq)
   return MBM_OK;
 }
 
-/* the captured attempt for DATA in buffer iD and NEW in buffer iN */
-int step_graph(mbm_ctx *c, int iD, int iN, int iT)
+/* the captured attempts (1, or GRAPH_ATTEMPTS with long) for DATA in buffer
+ * iD and NEW in buffer iN; the buffers of each attempt follow the deck
+ * states on the device, so one graph serves any number of attempts */
+int step_graph(mbm_ctx *c, int iD, int iN, int iT, bool lng)
 {
-  if (c->graph_ok[iD][iN]) return MBM_OK;
+  bool &ok = lng ? c->graphk_ok[iD][iN] : c->graph_ok[iD][iN];
+  if (ok) return MBM_OK;
+  hipGraphExec_t &ge = lng ? c->graphk[iD][iN] : c->graph[iD][iN];
   double *buf0 = c->layer[iD], *buf1 = c->layer[iN], *tmp = c->layer[iT];
+  const int n = lng ? GRAPH_ATTEMPTS : 1;
   int rc;
   if (c->ngroups == 2)   /* group 0: stage 1, stage 2, controller */
-    rc = capture(c, c->stream, &c->graph[iD][iN], [&] {
-      int r = enqueue_group_stage(c, 0, 1, buf0, buf1, tmp);
-      if (!r) r = enqueue_group_stage(c, 0, 2, buf0, buf1, tmp);
-      return r ? r : enqueue_group_control(c, 0, buf0, buf1, tmp);
+    rc = capture(c, c->stream, &ge, [&] {
+      int r = 0;
+      for (int i = 0; i < n && !r; i++) {
+        r = enqueue_group_stage(c, 0, 1, buf0, buf1, tmp);
+        if (!r) r = enqueue_group_stage(c, 0, 2, buf0, buf1, tmp);
+        if (!r) r = enqueue_group_control(c, 0, buf0, buf1, tmp);
+      }
+      return r;
     });
   else
-    rc = capture(c, c->stream, &c->graph[iD][iN],
-                 [&] { return enqueue_device_step(c, buf0, buf1, tmp); });
+    rc = capture(c, c->stream, &ge, [&] {
+      int r = 0;
+      for (int i = 0; i < n && !r; i++) r = enqueue_device_step(c, buf0, buf1, tmp);
+      return r;
+    });
   if (rc) return rc;
-  c->graph_ok[iD][iN] = true;
+  ok = true;
+  return MBM_OK;
+}
+
+/* group 1 of a two-group batch, half an attempt later: stage 2,
+ * controller, next stage 1 (1 or GRAPH_ATTEMPTS times) */
+int group1_graph(mbm_ctx *c, double *buf0, double *buf1, double *tmp, bool lng)
+{
+  bool &ok = lng ? c->graph2k_ok : c->graph2_ok;
+  if (ok) return MBM_OK;
+  const int n = lng ? GRAPH_ATTEMPTS : 1;
+  int rc = capture(c, c->stream2, lng ? &c->graph2k : &c->graph2, [&] {
+    int r = 0;
+    for (int i = 0; i < n && !r; i++) {
+      r = enqueue_group_stage(c, 1, 2, buf0, buf1, tmp);
+      if (!r) r = enqueue_group_control(c, 1, buf0, buf1, tmp);
+      if (!r) r = enqueue_group_stage(c, 1, 1, buf0, buf1, tmp);
+    }
+    return r;
+  });
+  if (rc) return rc;
+  ok = true;
   return MBM_OK;
 }
 
@@ -1475,20 +1510,13 @@ int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known)
   double *buf0 = c->layer[iD], *buf1 = c->layer[iN], *tmp = c->layer[iT];
   const bool two = c->ngroups == 2;
   int rc;
-  if ((rc = step_graph(c, iD, iN, iT))) return rc;
-  /* a plain context swaps the DATA and NEW roles after an odd number of
-   * accepted steps: the graph of the swapped assignment is captured now, so
-   * that no later call captures one between its launches */
-  if (!c->batch && (rc = step_graph(c, iN, iD, iT))) return rc;
-  if (two && !c->graph2_ok) {
-    /* group 1, half an attempt later: stage 2, controller, next stage 1 */
-    rc = capture(c, c->stream2, &c->graph2, [&] {
-      int r = enqueue_group_stage(c, 1, 2, buf0, buf1, tmp);
-      if (!r) r = enqueue_group_control(c, 1, buf0, buf1, tmp);
-      return r ? r : enqueue_group_stage(c, 1, 1, buf0, buf1, tmp);
-    });
-    if (rc) return rc;
-    c->graph2_ok = true;
+  for (int lng = 0; lng < 2; lng++) {
+    if ((rc = step_graph(c, iD, iN, iT, lng))) return rc;
+    /* a plain context swaps the DATA and NEW roles after an odd number of
+     * accepted steps: the graphs of the swapped assignment are captured now,
+     * so that no later call captures one between its launches */
+    if (!c->batch && (rc = step_graph(c, iN, iD, iT, lng))) return rc;
+    if (two && (rc = group1_graph(c, buf0, buf1, tmp, lng))) return rc;
   }
   std::vector<DeckState> st(c->ndecks);
   if (known) {
@@ -1519,10 +1547,14 @@ int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known)
         remaining = std::max(remaining, std::max(r, 1));
       }
     if (remaining == 0) break;
+    /* whole runs of GRAPH_ATTEMPTS, then single attempts: no attempt is
+     * launched beyond the estimate */
     const int chunk = std::min(remaining, maxchunk);
-    for (int i = 0; i < chunk; i++) {
-      HIPCHK(hipGraphLaunch(c->graph[iD][iN], c->stream));
-      if (two) HIPCHK(hipGraphLaunch(c->graph2, c->stream2));
+    for (int i = 0; i < chunk; ) {
+      const bool lng = chunk - i >= GRAPH_ATTEMPTS;
+      HIPCHK(hipGraphLaunch(lng ? c->graphk[iD][iN] : c->graph[iD][iN], c->stream));
+      if (two) HIPCHK(hipGraphLaunch(lng ? c->graph2k : c->graph2, c->stream2));
+      i += lng ? GRAPH_ATTEMPTS : 1;
     }
     if (two) {
       HIPCHK(hipEventRecord(c->grp_ev, c->stream2));

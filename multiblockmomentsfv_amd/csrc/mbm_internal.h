@@ -188,9 +188,10 @@ struct mbm_ctx {
   int reccap;
   bool batch;                     /* created by mbm_batch_create: roles never rotate */
   std::vector<int> deck_parity;   /* batches: DATA of deck d is buffer role[DATA] or role[NEW] */
-  /* captured Heun step + controller, per (DATA, NEW) buffer assignment */
-  hipGraphExec_t graph[3][3];
-  bool graph_ok[3][3];
+  /* captured Heun step + controller, per (DATA, NEW) buffer assignment:
+   * one attempt (graph) and GRAPH_ATTEMPTS attempts in a row (graphk) */
+  hipGraphExec_t graph[3][3], graphk[3][3];
+  bool graph_ok[3][3], graphk_ok[3][3];
   /* batches of two or more decks: decks [grp_deck[g], grp_deck[g+1]) form
    * group g (g = 0, 1), with tiles [grp_tile[g], grp_tile[g+1]) and ghosts
    * [grp_ghost[g], grp_ghost[g+1]). Group 1 runs on stream2, its attempt
@@ -199,9 +200,13 @@ struct mbm_ctx {
   int grp_deck[3], grp_tile[3], grp_ghost[3];
   hipStream_t stream2;
   hipEvent_t grp_ev;
-  hipGraphExec_t graph2;
-  bool graph2_ok;
+  hipGraphExec_t graph2, graph2k;
+  bool graph2_ok, graph2k_ok;
 };
+
+/* attempts per launch of the long step graphs: one graph launch costs the
+ * device a few microseconds more than a kernel boundary inside a graph */
+constexpr int GRAPH_ATTEMPTS = 8;
 
 namespace mbmi {
 std::vector<GhostDev> build_ghost_map(const std::vector<mbm_block_desc> &D,
