@@ -1,7 +1,9 @@
 # MultiBlockMomentsFV on MI355X -- build
 #
-#   make            multiblockmomentsfv_amd/libmbmfv.so (engine + host C) and
+#   make            multiblockmomentsfv_amd/libmbmfv.so (engine + host C),
 #                   build/moments (drop-in for the reference's ./build/moments)
+#                   and multiblockmomentsfv_amd/libmbmprobe.so (FP64 issue-rate
+#                   probe for bench.py --valu-rate)
 #   make oracle     the CPU oracle used by the tests (oracle/liboracle.so)
 #   make physcheck  host build of the engine's per-cell functions and their
 #                   VJPs for the adjoint tests (tests/native/libphyscheck.so)
@@ -24,6 +26,7 @@ HIPFLAGS := --offload-arch=$(ARCH) -O3 -std=c++17 -fPIC -Wall -Iinclude -ffp-con
 CFLAGS   := -O2 -fPIC -Wall -Iinclude -DMBM_GIT_COMMIT=\"$(GIT_COMMIT)\"
 
 LIB     := $(PKG)/libmbmfv.so
+PROBE   := $(PKG)/libmbmprobe.so
 BIN     := build/moments
 OBJDIR  := $(PKG)/_obj
 
@@ -34,7 +37,10 @@ CXX     ?= g++
 PHYSCHK := tests/native/libphyscheck.so
 
 .PHONY: all oracle physcheck clean
-all: $(LIB) $(BIN)
+all: $(LIB) $(BIN) $(PROBE)
+
+$(PROBE): $(PKG)/csrc/probe/fp64_rate.hip
+	$(HIPCC) $(HIPFLAGS) -shared -o $@ $<
 
 physcheck: $(PHYSCHK)
 
@@ -61,4 +67,4 @@ oracle:
 	$(MAKE) -C oracle
 
 clean:
-	rm -rf $(OBJDIR) $(LIB) build $(PHYSCHK)
+	rm -rf $(OBJDIR) $(LIB) $(PROBE) build $(PHYSCHK)

@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--deck PATH] [--no-cpu]
                     [--no-batched] [--no-adjoint]
-    python bench.py --calibration-copy | --stream-rates
+    python bench.py --calibration-copy | --stream-rates | --valu-rate
 
 Workload: square4 topology with every block 1024x1024 (4 194 304 cells),
 Reg13, Kn=0.01, CFL=0.9, harmonic limiter, Riemann2D initial state, FP64,
@@ -237,6 +237,33 @@ def calibration_copy():
     print(json.dumps({"calibration_copy_bytes": x.numel() * 8, "copies": 5}))
 
 
+def valu_rate():
+    """FP64 VALU issue rate of this GPU (libmbmprobe.so): wall time and shader
+    cycles per wave-instruction per SIMD for independent FP64 fma, mul, add
+    and a mix of the three, at 1, 2 and 4 resident waves per SIMD, and the
+    shader clock under that load. The stage kernels run two waves per SIMD."""
+    import ctypes
+    import torch  # noqa: F401  (same HIP runtime as the engine)
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = os.path.join(here, "multiblockmomentsfv_amd", "libmbmprobe.so")
+    if not os.path.exists(path):
+        raise SystemExit(f"{path} is not built: run `make` at the repository root")
+    L = ctypes.CDLL(path)
+    d = ctypes.c_double
+    L.mbm_probe_fp64_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(d), ctypes.POINTER(d),
+                                      ctypes.POINTER(d)]
+    out = {}
+    for waves in (1, 2, 4):
+        for op, name in enumerate(("fma", "mul", "add", "mix")):
+            ns, cyc, ghz = d(), d(), d()
+            rc = L.mbm_probe_fp64_rate(op, waves, ctypes.byref(ns), ctypes.byref(cyc), ctypes.byref(ghz))
+            if rc:
+                raise SystemExit(f"mbm_probe_fp64_rate failed ({rc})")
+            out[f"{name}_{waves}w"] = {"ns_per_instr": round(ns.value, 4), "cycles_per_instr": round(cyc.value, 3),
+                                       "clock_GHz": round(ghz.value, 3)}
+    print(json.dumps({"fp64_valu_issue_per_simd": out}))
+
+
 def stream_rates():
     """Streaming HBM rates of plain torch kernels: a copy (one read, one write)
     and an add (two reads, one write, the byte mix of stage 2), at the size of
@@ -284,7 +311,12 @@ def main():
                     help="only run the 256 MiB reference copies for the PMC calibration")
     ap.add_argument("--stream-rates", action="store_true",
                     help="only measure the streaming copy/add rates")
+    ap.add_argument("--valu-rate", action="store_true",
+                    help="only measure the FP64 VALU issue rate per SIMD")
     args = ap.parse_args()
+    if args.valu_rate:
+        valu_rate()
+        return
     if args.calibration_copy:
         calibration_copy()
         return

@@ -32,6 +32,16 @@ def test_library_exports_every_declared_symbol(mbm):
     assert len(declared_functions()) >= 30
 
 
+def test_fp64_rate_probe_is_built():
+    """bench.py --valu-rate loads the FP64 issue-rate probe; it is built with
+    the library and exports its one entry point (no GPU call here)."""
+    path = os.path.join(ROOT, "multiblockmomentsfv_amd", "libmbmprobe.so")
+    assert os.path.exists(path), "run make"
+    import torch  # noqa: F401  (the probe links the same HIP runtime as torch)
+    L = ctypes.CDLL(path)
+    assert hasattr(L, "mbm_probe_fp64_rate")
+
+
 @pytest.mark.parametrize("ny,band", [(1024, 37), (64, 20), (5, 2), (100, 1), (7, 4096), (2048, 45)])
 def test_row_order_keeps_marching_rows_adjacent(mbm, ny, band):
     """The device row order is a bijection of the ny+6 rows onto positions,
