@@ -104,7 +104,8 @@ def batched_leg(ndecks, steps, plain_steps, seed=0):
     decks = m.sweep_decks(ndecks, seed=seed)
     bat = m.Batch(decks)
     bat.init()
-    best, best_ms = bat.autotune(rows=(12, 16, 20, 24, 28, 32, 40), reps=4)
+    best, best_w, best_ms = bat.autotune_shape(rows=(16, 24, 32, 40, 48, 64, 80, 96), waves=(1, 2, 4, 8),
+                                               reps=4)
     bat.init()
     cells = np.array([sum(b["nx"] * b["ny"] for b in d["blocks"]) for d in decks], dtype=np.float64)
     torch.cuda.synchronize()
@@ -130,7 +131,7 @@ def batched_leg(ndecks, steps, plain_steps, seed=0):
     plain = plain_cu / plain_s
     return {"decks": ndecks, "cells": int(cells.sum()), "steps_per_deck": steps,
             "attempts": int((ns + nr).sum()), "value": batched, "unit": "cell-updates/s",
-            "seconds": wall, "tile_rows": best,
+            "seconds": wall, "tile_rows": best, "wave_group": best_w,
             "plain_per_deck": plain, "plain_steps_per_deck": plain_steps,
             "ratio_vs_plain": batched / plain,
             "config": f"seed {seed}: square4 topology, block sides from {{64,...,512}}, "

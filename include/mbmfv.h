@@ -89,6 +89,11 @@ int mbm_set_params(mbm_ctx *c, const mbm_params *p);          /* CLI overrides, 
  * chooses it from the deck and the device's compute-unit count. */
 int mbm_set_tile_rows(mbm_ctx *c, int rows);
 int mbm_tile_rows(const mbm_ctx *c);                          /* height in use */
+/* Waves (tiles) per workgroup of the stage kernels: 1, 2, 4 (the default) or
+ * 8. No counterpart in the reference (a launch-shape choice); results do not
+ * depend on it. Drops captured step graphs. */
+int mbm_set_wave_group(mbm_ctx *c, int waves);
+int mbm_wave_group(const mbm_ctx *c);
 
 /* Initial state on interior cells then the ghost fill (Main.c:124-138). */
 int mbm_init(mbm_ctx *c);
@@ -227,6 +232,11 @@ int mbm_batch_download(mbm_ctx *c, int deck, int block, double *host);
  * writes TMP and NEW, and drops a recorded adjoint run. */
 int mbm_batch_autotune(mbm_ctx *c, const int *rows, int nrows, int reps, int *best,
                        double *best_ms);
+/* As mbm_batch_autotune over every pair of a candidate tile height and a
+ * candidate wave group (1, 2, 4 or 8 waves per workgroup); keeps the fastest
+ * pair. */
+int mbm_batch_autotune_shape(mbm_ctx *c, const int *rows, int nrows, const int *waves, int nwaves,
+                             int reps, int *best_rows, int *best_waves, double *best_ms);
 
 #ifdef __cplusplus
 }

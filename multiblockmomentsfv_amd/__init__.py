@@ -100,6 +100,8 @@ def lib():
         "mbm_set_params": (i, [p, ctypes.POINTER(Params)]),
         "mbm_set_tile_rows": (i, [p, i]),
         "mbm_tile_rows": (i, [p]),
+        "mbm_set_wave_group": (i, [p, i]),
+        "mbm_wave_group": (i, [p]),
         "mbm_init": (i, [p]),
         "mbm_upload": (i, [p, i, i, p]),
         "mbm_download": (i, [p, i, i, p]),
@@ -133,6 +135,8 @@ def lib():
         "mbm_batch_run": (i, [p, p, p, d, i, p, p, p]),
         "mbm_batch_download": (i, [p, i, i, p]),
         "mbm_batch_autotune": (i, [p, p, i, i, ctypes.POINTER(i), ctypes.POINTER(d)]),
+        "mbm_batch_autotune_shape": (i, [p, p, i, p, i, i, ctypes.POINTER(i), ctypes.POINTER(i),
+                                         ctypes.POINTER(d)]),
         "mbm_host_set_boundary": (i, [ctypes.POINTER(BlockDesc), i, ctypes.POINTER(Params),
                                       ctypes.POINTER(p), d]),
         "mbm_host_last_error": (ctypes.c_char_p, []),
@@ -161,7 +165,8 @@ def lib():
 def exported_symbols():
     """Names the C headers declare (checked against the library by the tests)."""
     return [n for n in ("mbm_last_error", "mbm_create", "mbm_destroy", "mbm_set_stream",
-                        "mbm_set_params", "mbm_set_tile_rows", "mbm_tile_rows", "mbm_init", "mbm_upload",
+                        "mbm_set_params", "mbm_set_tile_rows", "mbm_tile_rows", "mbm_set_wave_group",
+                        "mbm_wave_group", "mbm_init", "mbm_upload",
                         "mbm_download", "mbm_layer_ptr", "mbm_layer_rows", "mbm_row_order",
                         "mbm_set_boundary",
                         "mbm_time_update", "mbm_commit", "mbm_steps", "mbm_output_values",
@@ -169,7 +174,8 @@ def exported_symbols():
                         "mbm_kernel_time", "mbm_layer_elems", "mbm_layer_get", "mbm_layer_set",
                         "mbm_adj_forward", "mbm_adj_schedule", "mbm_adj_backward",
                         "mbm_adj_release", "mbm_batch_create", "mbm_batch_ndecks",
-                        "mbm_batch_run", "mbm_batch_download", "mbm_batch_autotune")]
+                        "mbm_batch_run", "mbm_batch_download", "mbm_batch_autotune",
+                        "mbm_batch_autotune_shape")]
 
 
 def _check(rc, what):
@@ -332,6 +338,13 @@ class Engine:
 
     def tile_rows(self):
         return lib().mbm_tile_rows(self.h)
+
+    def set_wave_group(self, waves):
+        """Waves (tiles) per workgroup of the stage kernels: 1, 2, 4 or 8."""
+        _check(lib().mbm_set_wave_group(self.h, int(waves)), "mbm_set_wave_group")
+
+    def wave_group(self):
+        return lib().mbm_wave_group(self.h)
 
     def init(self):
         _check(lib().mbm_init(self.h), "mbm_init")
@@ -584,6 +597,23 @@ class Batch:
         _check(lib().mbm_batch_autotune(self.h, _ptr(r), len(r), reps, ctypes.byref(best),
                                         ctypes.byref(ms)), "mbm_batch_autotune")
         return best.value, ms.value
+
+    def autotune_shape(self, rows=(16, 24, 32, 40, 48, 64, 80, 96), waves=(1, 2, 4, 8), reps=3):
+        """Tile height and wave group of the fastest stage pair; returns
+        (rows, waves, ms per stage pair)."""
+        r = np.array(rows, dtype=np.int32)
+        w = np.array(waves, dtype=np.int32)
+        br, bw, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().mbm_batch_autotune_shape(self.h, _ptr(r), len(r), _ptr(w), len(w), reps,
+                                              ctypes.byref(br), ctypes.byref(bw), ctypes.byref(ms)),
+               "mbm_batch_autotune_shape")
+        return br.value, bw.value, ms.value
+
+    def set_wave_group(self, waves):
+        _check(lib().mbm_set_wave_group(self.h, int(waves)), "mbm_set_wave_group")
+
+    def wave_group(self):
+        return lib().mbm_wave_group(self.h)
 
     def run(self, maxsteps, tend=None, cflmax=1.0):
         n = len(self.decks)

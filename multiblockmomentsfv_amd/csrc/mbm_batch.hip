@@ -137,13 +137,16 @@ extern "C" int mbm_batch_download(mbm_ctx *c, int deck, int block, double *host)
 
 extern "C" int mbm_batch_ndecks(const mbm_ctx *c) { return c ? c->ndecks : 0; }
 
-/* Times 'reps' stage pairs of every deck (at the decks' current dt) for
- * each candidate tile height and keeps the fastest. The state is not
- * changed: the stages write TMP and the NEW side only. */
-extern "C" int mbm_batch_autotune(mbm_ctx *c, const int *rows, int nrows, int reps, int *best,
-                                  double *best_ms)
+/* Times 'reps' stage pairs (with their fills) of every deck, at the decks'
+ * current dt, for each pair of a candidate wave group and tile height, and
+ * keeps the fastest pair. The state is not changed: the stages write TMP and
+ * the NEW side only. */
+extern "C" int mbm_batch_autotune_shape(mbm_ctx *c, const int *rows, int nrows, const int *waves,
+                                        int nwaves, int reps, int *best_rows, int *best_waves,
+                                        double *best_ms)
 {
-  if (!c || !rows || nrows < 1 || reps < 1) return set_err(MBM_E_ARG, "invalid argument");
+  if (!c || !rows || nrows < 1 || !waves || nwaves < 1 || reps < 1)
+    return set_err(MBM_E_ARG, "invalid argument");
   if (c->deck_parity.size() != (size_t)c->ndecks) c->deck_parity.assign(c->ndecks, 0);
   std::vector<DeckState> st(c->ndecks);
   for (int d = 0; d < c->ndecks; d++) {
@@ -159,27 +162,41 @@ extern "C" int mbm_batch_autotune(mbm_ctx *c, const int *rows, int nrows, int re
   }
   HIPCHK(hipMemcpyAsync(c->ddecks, st.data(), st.size() * sizeof(DeckState), hipMemcpyHostToDevice,
                         c->stream));
-  int bestr = c->tile_rows, rc;
+  int bestr = c->tile_rows, bestw = c->wpg, rc;
   double bestt = 1e300;
-  for (int i = 0; i < nrows; i++) {
-    if ((rc = mbm_set_tile_rows(c, rows[i]))) return rc;
-    if ((rc = enqueue_device_stages(c))) return rc;    /* warm-up */
-    HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    for (int r = 0; r < reps; r++)
-      if ((rc = enqueue_device_stages(c))) return rc;
-    HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(hipEventSynchronize(c->ev[1]));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    if (ms / reps < bestt) {
-      bestt = ms / reps;
-      bestr = rows[i];
+  for (int k = 0; k < nwaves; k++) {
+    if ((rc = mbm_set_wave_group(c, waves[k]))) return rc;
+    for (int i = 0; i < nrows; i++) {
+      if ((rc = mbm_set_tile_rows(c, rows[i]))) return rc;
+      if ((rc = enqueue_device_stages(c))) return rc;    /* warm-up */
+      HIPCHK(hipEventRecord(c->ev[0], c->stream));
+      for (int r = 0; r < reps; r++)
+        if ((rc = enqueue_device_stages(c))) return rc;
+      HIPCHK(hipEventRecord(c->ev[1], c->stream));
+      HIPCHK(hipEventSynchronize(c->ev[1]));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+      if (ms / reps < bestt) {
+        bestt = ms / reps;
+        bestr = rows[i];
+        bestw = waves[k];
+      }
     }
   }
   if ((rc = clear_slots(c))) return rc;
+  if ((rc = mbm_set_wave_group(c, bestw))) return rc;
   if ((rc = mbm_set_tile_rows(c, bestr))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (best) *best = bestr;
+  if (best_rows) *best_rows = bestr;
+  if (best_waves) *best_waves = bestw;
   if (best_ms) *best_ms = bestt;
   return MBM_OK;
+}
+
+extern "C" int mbm_batch_autotune(mbm_ctx *c, const int *rows, int nrows, int reps, int *best,
+                                  double *best_ms)
+{
+  if (!c) return set_err(MBM_E_ARG, "invalid argument");
+  const int w = c->wpg;
+  return mbm_batch_autotune_shape(c, rows, nrows, &w, 1, reps, best, nullptr, best_ms);
 }

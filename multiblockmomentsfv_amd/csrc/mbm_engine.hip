@@ -540,16 +540,24 @@ __device__ __forceinline__ int tile_pass(const StageArgs &A, int tix, double (*s
   return (anyslow || mzero != 0ull) ? 2 : 1;
 }
 
+/* bytes of dynamic LDS a stage workgroup of w waves needs (per wave: the
+ * carried face state and the initial values of two rows, 3 x 9 x 64 doubles) */
+constexpr size_t stage_lds_bytes(int w) { return (size_t)w * 3 * NV * 64 * sizeof(double); }
+
+/* One workgroup = blockDim.x / 64 waves, each running consecutive tiles of
+ * the launch order (the context's wave group, 1 to MAX_WAVES_PER_WG). */
 template <int STAGE, int LIM, bool DOWN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MBM_STAGE_WAVES, MBM_STAGE_WAVES)))
+__global__ void __launch_bounds__(64 * MAX_WAVES_PER_WG)
+__attribute__((amdgpu_waves_per_eu(MBM_STAGE_WAVES, MBM_STAGE_WAVES)))
 stage_kernel(const StageArgs A)
 {
   /* the production CFL (stage 1) is folded by the upward march only */
   static_assert(!DOWN || STAGE == 2, "downward march is stage 2 only");
   const int lane = threadIdx.x & 63;
-  const int tix = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES_PER_WG + (threadIdx.x >> 6));
+  const int tix = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   if (tix >= A.ntiles) return;
-  __shared__ double sm[WAVES_PER_WG][3][NV][64];
+  extern __shared__ double stage_lds[];
+  double (*sm)[3][NV][64] = reinterpret_cast<double (*)[3][NV][64]>(stage_lds);
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   TileOut o;
   const int pass = tile_pass<STAGE, LIM, DOWN, false>(A, tix, sm[wv], o);
@@ -983,27 +991,27 @@ std::vector<TileDev> build_tiles(const std::vector<mbm_block_desc> &D, int rows)
   return t;
 }
 
-/* Launch order of the tiles. A workgroup runs WAVES_PER_WG consecutive tiles
+/* Launch order of the tiles. A workgroup runs w (the wave group) consecutive tiles
  * (adjacent strips: they share the cache lines at strip edges and the halo
  * columns), and workgroup w goes to XCD w mod 8. The groups are dealt so
  * that each XCD receives one contiguous run of groups -- a patch of
  * neighbouring strips and row ranges -- keeping halo and edge lines in that
  * XCD's L2. Padding entries (blk = -1) exit at once. */
-std::vector<TileDev> xcd_order(const std::vector<TileDev> &t)
+std::vector<TileDev> xcd_order(const std::vector<TileDev> &t, int w)
 {
   constexpr int NXCD = 8;
-  const int G = (int)((t.size() + WAVES_PER_WG - 1) / WAVES_PER_WG);
+  const int G = (int)((t.size() + w - 1) / w);
   const int per = (G + NXCD - 1) / NXCD;
-  std::vector<TileDev> o((size_t)per * NXCD * WAVES_PER_WG);
+  std::vector<TileDev> o((size_t)per * NXCD * w);
   for (auto &d : o) {
     d.blk = -1;
     d.a0 = d.r0 = d.r1 = 0;
   }
   for (int g = 0; g < G; g++) {
     const int x = g / per, j = g % per, wg = j * NXCD + x;
-    for (int w = 0; w < WAVES_PER_WG; w++) {
-      const size_t src = (size_t)g * WAVES_PER_WG + w;
-      if (src < t.size()) o[(size_t)wg * WAVES_PER_WG + w] = t[src];
+    for (int k = 0; k < w; k++) {
+      const size_t src = (size_t)g * w + k;
+      if (src < t.size()) o[(size_t)wg * w + k] = t[src];
     }
   }
   return o;
@@ -1057,13 +1065,13 @@ int upload_tiles(mbm_ctx *c)
     const int b1 = c->deck_first[c->grp_deck[1]];
     std::vector<TileDev> t0, t1;
     for (const TileDev &d : all) (d.blk < b1 ? t0 : t1).push_back(d);
-    t = xcd_order(t0);
+    t = xcd_order(t0, c->wpg);
     grp[1] = (int)t.size();
-    const std::vector<TileDev> o1 = xcd_order(t1);
+    const std::vector<TileDev> o1 = xcd_order(t1, c->wpg);
     t.insert(t.end(), o1.begin(), o1.end());
     grp[2] = (int)t.size();
   } else {
-    t = xcd_order(build_tiles(c->desc, rows));
+    t = xcd_order(build_tiles(c->desc, rows), c->wpg);
   }
   /* the new tile list is on the device before anything of the context
    * changes: a failure leaves tiles, layout and tile height as they were */
@@ -1092,20 +1100,51 @@ int upload_tiles(mbm_ctx *c)
   return MBM_OK;
 }
 
+/* raises the dynamic LDS limit of every stage kernel instance to the largest
+ * wave group (on the current device; called when a context is created) */
+template <int STAGE, int LIM>
+hipError_t raise_lds_limit()
+{
+  constexpr bool down = STAGE == 2 && MBM_STAGE2_DOWN;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&stage_kernel<STAGE, LIM, down>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)stage_lds_bytes(MAX_WAVES_PER_WG));
+}
+
+int raise_stage_lds_limits()
+{
+  const hipError_t e[] = { raise_lds_limit<1, 0>(), raise_lds_limit<1, 1>(), raise_lds_limit<1, 2>(),
+                           raise_lds_limit<1, 3>(), raise_lds_limit<1, 4>(), raise_lds_limit<2, 0>(),
+                           raise_lds_limit<2, 1>(), raise_lds_limit<2, 2>(), raise_lds_limit<2, 3>(),
+                           raise_lds_limit<2, 4>() };
+  for (hipError_t x : e)
+    if (x != hipSuccess)
+      return set_err(MBM_E_HIP, "raising the LDS limit of the stage kernels failed: %s", hipGetErrorString(x));
+  return MBM_OK;
+}
+
+/* launches one stage kernel instance with the context's wave group */
+template <int STAGE, int LIM>
+int launch_stage_lim(mbm_ctx *c, const StageArgs &A)
+{
+  constexpr bool down = STAGE == 2 && MBM_STAGE2_DOWN;
+  const int w = c->wpg;
+  const dim3 grid((A.ntiles + w - 1) / w), block(64 * w);
+  hipLaunchKernelGGL((stage_kernel<STAGE, LIM, down>), grid, block, stage_lds_bytes(w), c->stream, A);
+  HIPCHK(hipGetLastError());
+  return MBM_OK;
+}
+
 template <int STAGE>
 int launch_stage(mbm_ctx *c, const StageArgs &A)
 {
-  const dim3 grid((A.ntiles + WAVES_PER_WG - 1) / WAVES_PER_WG), block(64 * WAVES_PER_WG);
-  constexpr bool down = STAGE == 2 && MBM_STAGE2_DOWN;
   switch (c->par.limiter) {
-    case 0: hipLaunchKernelGGL((stage_kernel<STAGE, 0, down>), grid, block, 0, c->stream, A); break;
-    case 1: hipLaunchKernelGGL((stage_kernel<STAGE, 1, down>), grid, block, 0, c->stream, A); break;
-    case 2: hipLaunchKernelGGL((stage_kernel<STAGE, 2, down>), grid, block, 0, c->stream, A); break;
-    case 3: hipLaunchKernelGGL((stage_kernel<STAGE, 3, down>), grid, block, 0, c->stream, A); break;
-    default: hipLaunchKernelGGL((stage_kernel<STAGE, 4, down>), grid, block, 0, c->stream, A); break;
+    case 0: return launch_stage_lim<STAGE, 0>(c, A);
+    case 1: return launch_stage_lim<STAGE, 1>(c, A);
+    case 2: return launch_stage_lim<STAGE, 2>(c, A);
+    case 3: return launch_stage_lim<STAGE, 3>(c, A);
+    default: return launch_stage_lim<STAGE, 4>(c, A);
   }
-  HIPCHK(hipGetLastError());
-  return MBM_OK;
 }
 
 int fill_buffer(mbm_ctx *c, double *layer)
@@ -1264,6 +1303,7 @@ mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_f
   c->desc.assign(blocks, blocks + nblocks);
   c->par = deck_par[0];
   c->tile_rows = 0;                      /* automatic */
+  c->wpg = WAVES_PER_WG;
   c->cus = 256;
   {
     int dev = 0, cus = 0;
@@ -1276,6 +1316,7 @@ mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_f
   c->deck_par.assign(deck_par, deck_par + ndecks);
   if (!hipok(hipGetDevice(&c->device), "hipGetDevice")) return fail(rc);
   if ((rc = resolve_scratch(c->device))) return fail(rc);
+  if ((rc = raise_stage_lds_limits())) return fail(rc);
   for (int d = 0; d < ndecks; d++)
     for (int k = deck_first[d]; k < deck_first[d + 1]; k++) {
       BlockDev b{};
@@ -1646,6 +1687,24 @@ extern "C" int mbm_set_tile_rows(mbm_ctx *c, int rows)
   if (rc) c->tile_rows = keep;   /* the layers are unchanged when the move failed */
   return rc;
 }
+
+extern "C" int mbm_set_wave_group(mbm_ctx *c, int waves)
+{
+  if (!c) return set_err(MBM_E_ARG, "NULL context");
+  if (waves != 1 && waves != 2 && waves != 4 && waves != MAX_WAVES_PER_WG)
+    return set_err(MBM_E_ARG, "wave group %d not one of 1, 2, 4, %d", waves, MAX_WAVES_PER_WG);
+  if (waves == c->wpg) return MBM_OK;
+  const int keep = c->wpg;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->stream2) HIPCHK(hipStreamSynchronize(c->stream2));
+  release_graphs(c);
+  c->wpg = waves;
+  const int rc = upload_tiles(c);
+  if (rc) c->wpg = keep;         /* the tile list is unchanged when the upload failed */
+  return rc;
+}
+
+extern "C" int mbm_wave_group(const mbm_ctx *c) { return c ? c->wpg : 0; }
 
 extern "C" int mbm_init(mbm_ctx *c)
 {

@@ -34,7 +34,11 @@ constexpr int STRIP = 60;          /* owned columns per wavefront */
  * production CFL (bit patterns of non-negative doubles, NaN ignored), and 1 +
  * the scan position of the last NaN production CFL (0: none) */
 constexpr int SLOTS_PER_BLOCK = 4;
+/* waves (tiles) per workgroup of the stage kernels: the default, and the
+ * largest a context may choose (8 waves hold 108 KiB of LDS, one workgroup
+ * per CU) */
 constexpr int WAVES_PER_WG = 4;
+constexpr int MAX_WAVES_PER_WG = 8;
 
 /* A block in a layer buffer. Rows are stored at row positions: a position
  * holds the 9 components of one row back to back (component m at m*pitch
@@ -163,6 +167,7 @@ struct mbm_ctx {
   int tile_rows;             /* requested tile height, 0 = automatic */
   int rows_used;             /* tile height of the current tile list */
   int cus;                   /* compute units of the device */
+  int wpg;                   /* waves per workgroup of the stage kernels (1, 2, 4, 8) */
   FillRec *dfill;             /* ghost fill records, in block order */
   double2 *dfillxy;           /* ghost cell centres (scenario ghosts) */
   int nghost;

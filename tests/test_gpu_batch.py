@@ -63,6 +63,54 @@ def test_batch_tend_and_autotune(mbm):
         assert rel_linf(bat.download(i), eng.download(0)).max() == 0.0
 
 
+def test_batch_autotune_shape_keeps_results(mbm):
+    """The shape autotuner picks a (tile height, wave group) pair from the
+    candidates, and the batch then advances exactly as the decks alone."""
+    decks = small_sweep(mbm, n=5, seed=2)
+    bat = mbm.Batch(decks)
+    bat.init()
+    rows, waves, ms = bat.autotune_shape(rows=(4, 16), waves=(1, 2, 8), reps=1)
+    assert rows in (4, 16) and waves in (1, 2, 8) and ms > 0
+    assert bat.wave_group() == waves
+    bat.init()
+    ns, nr = bat.run(maxsteps=12)
+    for i, d in enumerate(decks):
+        eng = mbm.Engine(d["blocks"], kn=d["kn"])
+        eng.init()
+        t, n, rej, _ = eng.steps(0.0, 1e30, d["cfl"], maxsteps=12)
+        assert n == ns[i] and rej == nr[i] and t == bat.t[i]
+        for g, r in zip(bat.download(i), eng.download(0)):
+            assert g.tobytes() == r.tobytes(), f"deck {i}"
+        eng.close()
+    bat.close()
+
+
+@pytest.mark.parametrize("waves", [1, 2, 8])
+def test_wave_group_does_not_change_results(mbm, waves):
+    """Waves per workgroup of the stage kernels is a launch shape only: the
+    step log and the state are bit-identical to the default group (4)."""
+    d = small_sweep(mbm, n=1, seed=7)[0]
+    out = []
+    for w in (4, waves):
+        eng = mbm.Engine(d["blocks"], kn=d["kn"])
+        eng.set_wave_group(w)
+        assert eng.wave_group() == w
+        eng.init()
+        t, n, rej, log = eng.steps(0.0, 1e30, d["cfl"], maxsteps=20)
+        out.append((t, n, rej, [a.tobytes() for a in eng.download(0)]))
+        eng.close()
+    assert out[0] == out[1]
+
+
+def test_wave_group_rejects_other_sizes(mbm):
+    d = small_sweep(mbm, n=1, seed=7)[0]
+    eng = mbm.Engine(d["blocks"], kn=d["kn"])
+    with pytest.raises(mbm.MbmError):
+        eng.set_wave_group(3)
+    assert eng.wave_group() == 4
+    eng.close()
+
+
 def test_full_size_sweep_subset_equals_single_runs(mbm):
     """The 256-deck sweep at its full sizes (block sides 64..512, 88 M cells)
     with an autotuned tile height: a few steps, then four decks against their
