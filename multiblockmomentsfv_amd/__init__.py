@@ -322,7 +322,7 @@ class Engine:
 
     def __del__(self):
         # at interpreter exit the module globals may already be gone; the
-        # process is ending and the driver releases the device memory
+        # process is ending and the GPU driver releases the device memory
         try:
             self.close()
         except Exception:
@@ -578,7 +578,7 @@ class Batch:
 
     def __del__(self):
         # at interpreter exit the module globals may already be gone; the
-        # process is ending and the driver releases the device memory
+        # process is ending and the GPU driver releases the device memory
         try:
             self.close()
         except Exception:
