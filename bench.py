@@ -40,6 +40,8 @@ METRIC = "FP64 cell-updates/s + % HBM roofline; batched & adjoint vs plain step"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 360.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_square4_1024_traffic.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "stage_kernel_square4_1024_valu.json")
+COMPULSORY_BYTES_PER_CELL_UPDATE = 144.0     # U^n in, U^n+1 out
 
 
 def cpu_model():
@@ -238,30 +240,36 @@ def calibration_copy():
     print(json.dumps({"calibration_copy_bytes": x.numel() * 8, "copies": 5}))
 
 
-def valu_rate():
-    """FP64 VALU issue rate of this GPU (libmbmprobe.so): wall time and shader
-    cycles per wave-instruction per SIMD for independent FP64 fma, mul, add
-    and a mix of the three, at 1, 2 and 4 resident waves per SIMD, and the
-    shader clock under that load. The stage kernels run two waves per SIMD."""
+def fp64_issue(op=3, waves=2, launches=200):
+    """FP64 VALU issue of this GPU (libmbmprobe.so): ns and shader cycles per
+    wave-instruction per SIMD of independent FP64 work over the last half of
+    `launches` back-to-back launches (about 70 us each), and the shader clock
+    under that load. op 0 fma, 1 mul, 2 add, 3 their mix; waves resident per
+    SIMD 1, 2 or 4 (the stage kernels run two)."""
     import ctypes
     import torch  # noqa: F401  (same HIP runtime as the engine)
-    here = os.path.dirname(os.path.abspath(__file__))
-    path = os.path.join(here, "multiblockmomentsfv_amd", "libmbmprobe.so")
+    path = os.path.join(ROOT, "multiblockmomentsfv_amd", "libmbmprobe.so")
     if not os.path.exists(path):
         raise SystemExit(f"{path} is not built: run `make` at the repository root")
     L = ctypes.CDLL(path)
     d = ctypes.c_double
-    L.mbm_probe_fp64_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(d), ctypes.POINTER(d),
-                                      ctypes.POINTER(d)]
+    L.mbm_probe_fp64_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(d),
+                                      ctypes.POINTER(d), ctypes.POINTER(d)]
+    ns, cyc, ghz = d(), d(), d()
+    rc = L.mbm_probe_fp64_rate(op, waves, launches, ctypes.byref(ns), ctypes.byref(cyc), ctypes.byref(ghz))
+    if rc:
+        raise RuntimeError(f"mbm_probe_fp64_rate failed ({rc})")
+    return ns.value, cyc.value, ghz.value
+
+
+def valu_rate():
+    """The probe over fma, mul, add and their mix at 1, 2 and 4 waves per SIMD."""
     out = {}
     for waves in (1, 2, 4):
         for op, name in enumerate(("fma", "mul", "add", "mix")):
-            ns, cyc, ghz = d(), d(), d()
-            rc = L.mbm_probe_fp64_rate(op, waves, ctypes.byref(ns), ctypes.byref(cyc), ctypes.byref(ghz))
-            if rc:
-                raise SystemExit(f"mbm_probe_fp64_rate failed ({rc})")
-            out[f"{name}_{waves}w"] = {"ns_per_instr": round(ns.value, 4), "cycles_per_instr": round(cyc.value, 3),
-                                       "clock_GHz": round(ghz.value, 3)}
+            ns, cyc, ghz = fp64_issue(op, waves)
+            out[f"{name}_{waves}w"] = {"ns_per_instr": round(ns, 4), "cycles_per_instr": round(cyc, 3),
+                                       "clock_GHz": round(ghz, 3)}
     print(json.dumps({"fp64_valu_issue_per_simd": out}))
 
 
@@ -365,9 +373,22 @@ def main():
     bytes_per_launch = cells * BYTES_PER_CELL_UPDATE / 2
     achieved = bytes_per_launch / mean_launch_s / 1e9
     traffic = None    # measured (rocprofv3 FETCH_SIZE + WRITE_SIZE) for the default deck only
-    if os.path.exists(TRAFFIC_FILE) and os.path.abspath(args.deck) == os.path.abspath(DECK):
+    valu = None       # FP64 issue: rocprofv3 SQ_INSTS_VALU per launch (default deck) and a live probe
+    default_deck = os.path.abspath(args.deck) == os.path.abspath(DECK)
+    if os.path.exists(TRAFFIC_FILE) and default_deck:
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    if os.path.exists(VALU_FILE) and default_deck:
+        with open(VALU_FILE) as f:
+            vf = json.load(f)
+        ns, cyc, ghz = fp64_issue()
+        floor_s = vf["insts_per_launch"] / vf["simds"] * ns * 1e-9
+        valu = {"insts_per_launch": vf["insts_per_launch"], "probe_ns_per_instr": ns,
+                "probe_cycles_per_instr": cyc, "probe_clock_GHz": ghz,
+                "floor_ms": floor_s * 1e3, "frac": floor_s / mean_launch_s,
+                "note": "SQ_INSTS_VALU per stage launch (rocprofv3, committed profile) at the FP64 "
+                        "issue rate the probe measures on this GPU under sustained load, two waves "
+                        "per SIMD (bench.py --valu-rate) = floor_ms; frac = floor_ms / mean_launch_ms"}
     if os.path.abspath(args.deck) == os.path.abspath(DECK):
         workload = ("square4 topology, 4 blocks x 1024x1024 (4194304 cells), Reg13, "
                     "Kn=0.01, CFL=0.9, harmonic, Riemann2D initial state, adaptive dt")
@@ -407,6 +428,8 @@ def main():
             "timed_launches": nstage,
             "fill_mean_launch_ms": fill_ms / max(nfill, 1),
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "compulsory_frac": cells * COMPULSORY_BYTES_PER_CELL_UPDATE / 2 / mean_launch_s / 1e9 / HBM_PEAK_GBS,
+            "valu": valu,
         },
     }
     eng.close()

@@ -67,14 +67,19 @@ hipError_t launch_op(int op, int wgs, double *sink, long long *clk, hipStream_t 
 } // namespace
 
 /* Runs op (0 fma, 1 mul, 2 add, 3 mixed) with waves_per_simd resident waves
- * on every SIMD of device 0 (1, 2 or 4). Returns 0 on success and sets:
+ * on every SIMD of the current device (1, 2 or 4), in `launches` back-to-back
+ * launches of which the last half are timed (the shader clock settles under
+ * sustained FP64 load within a few milliseconds; one launch takes about
+ * 70 us). Returns 0 on success and sets:
  *   ns_per_instr  wall time per wave-instruction per SIMD (nanoseconds),
- *   cycles_per_instr  shader-clock cycles per wave-instruction per SIMD,
- *   clock_ghz     mean shader clock during the run. */
-extern "C" int mbm_probe_fp64_rate(int op, int waves_per_simd, double *ns_per_instr,
+ *   cycles_per_instr  shader-clock cycles per wave-instruction per SIMD, as
+ *                 wall time times the clock,
+ *   clock_ghz     mean shader clock during the last launch. */
+extern "C" int mbm_probe_fp64_rate(int op, int waves_per_simd, int launches, double *ns_per_instr,
                                    double *cycles_per_instr, double *clock_ghz)
 {
-  if (op < 0 || op > 3 || (waves_per_simd != 1 && waves_per_simd != 2 && waves_per_simd != 4))
+  if (op < 0 || op > 3 || (waves_per_simd != 1 && waves_per_simd != 2 && waves_per_simd != 4) ||
+      launches < 2 || launches > 100000)
     return 1;
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -86,13 +91,14 @@ extern "C" int mbm_probe_fp64_rate(int op, int waves_per_simd, double *ns_per_in
   long long *clk = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = 3;
-  if (hipMalloc(&sink, (size_t)wgs * WG * sizeof(double)) == hipSuccess &&
-      hipMalloc(&clk, (size_t)2 * wgs * sizeof(long long)) == hipSuccess &&
-      hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
-      launch_op(op, wgs, sink, clk, nullptr) == hipSuccess &&      /* warm-up */
-      hipEventRecord(e0, nullptr) == hipSuccess &&
-      launch_op(op, wgs, sink, clk, nullptr) == hipSuccess &&
-      hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess) {
+  const int untimed = launches - launches / 2, timed = launches / 2;
+  bool ok = hipMalloc(&sink, (size_t)wgs * WG * sizeof(double)) == hipSuccess &&
+            hipMalloc(&clk, (size_t)2 * wgs * sizeof(long long)) == hipSuccess &&
+            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  for (int i = 0; ok && i < untimed; i++) ok = launch_op(op, wgs, sink, clk, nullptr) == hipSuccess;
+  ok = ok && hipEventRecord(e0, nullptr) == hipSuccess;
+  for (int i = 0; ok && i < timed; i++) ok = launch_op(op, wgs, sink, clk, nullptr) == hipSuccess;
+  if (ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess) {
     float ms = 0.f;
     long long *h = new long long[2 * (size_t)wgs];
     if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
@@ -104,10 +110,12 @@ extern "C" int mbm_probe_fp64_rate(int op, int waves_per_simd, double *ns_per_in
       }
       cyc /= wgs;
       rt /= wgs;
-      const double per_simd = (double)waves_per_simd * ITERS * NACC;   /* wave-instructions */
-      if (ns_per_instr) *ns_per_instr = ms * 1e6 / per_simd;
-      if (cycles_per_instr) *cycles_per_instr = cyc / per_simd;
-      if (clock_ghz) *clock_ghz = rt > 0 ? cyc / (rt * 10.0) : 0.0;   /* 100 MHz real-time counter */
+      const double per_simd = (double)timed * waves_per_simd * ITERS * NACC;   /* wave-instructions */
+      const double ghz = rt > 0 ? cyc / (rt * 10.0) : 0.0;          /* 100 MHz real-time counter */
+      const double ns = ms * 1e6 / per_simd;
+      if (ns_per_instr) *ns_per_instr = ns;
+      if (cycles_per_instr) *cycles_per_instr = ns * ghz;
+      if (clock_ghz) *clock_ghz = ghz;
       rc = 0;
     }
     delete[] h;
