@@ -278,18 +278,54 @@ MBM_DEV void cons_flux(const double *W, double *U, double *F)
   uf<6>(W, s, U[6], F[6]); uf<7>(W, s, U[7], F[7]); uf<8>(W, s, U[8], F[8]);
 }
 
+/* component n of the shifted flux g = F - lam U = w U + p of one side, with
+ * w = un - lam and p = F - un U, the part of the Grad-13 flux that is not
+ * carried by the normal velocity (uf above, regrouped):
+ *   p = (0, rho Tnn, rho Tnt, qn + rho (T.v)_n, 1.2 qn + 2 un rho Tnn,
+ *        0.4 qt + rho (Tnn ut + Tnt un), 0.4 qn + 2 ut rho Tnt,
+ *        un (2.2 qn + rho (T.v)_n) + 0.4 qt ut + rho (Tnn |v|^2 + R11) / 2,
+ *        1.4 qn ut + 0.4 qt un + rho (un ut Tnn + Tnt (un^2/2 + 3 ut^2/2 + 7 th/2)))
+ * with R11 = th (7 Tnn - 2 th). */
 template <int N>
-MBM_DEV void hll_comp(const double *WL, const Side &sL, const double *WR, const Side &sR,
-                      double ca, double cb, double cc, double *Fs)
+MBM_DEV double gshift(const double *W, const Side &s, double w)
 {
-  double uL, fL, uR, fR;
-  uf<N>(WL, sL, uL, fL);
-  uf<N>(WR, sR, uR, fR);
-  Fs[N] = ca * fL + cb * fR + cc * (uR - uL);
+  const double rho = W[0], un = W[1], ut = W[2], th = W[3];
+  const double tnn = W[4], tnt = W[5], ttt = W[6], qn = W[7], qt = W[8];
+  if constexpr (N == 0) {
+    return w * rho;
+  } else if constexpr (N == 1) {
+    return w * (rho * un) + rho * tnn;
+  } else if constexpr (N == 2) {
+    return w * (rho * ut) + rho * tnt;
+  } else if constexpr (N == 3) {
+    return w * s.E + (qn + rho * s.An);
+  } else if constexpr (N == 4) {
+    const double rt = rho * tnn;
+    return w * (rt + rho * s.un2) + (1.2 * qn + 2.0 * un * rt);
+  } else if constexpr (N == 5) {
+    return w * (rho * (tnt + s.unut)) + (0.4 * qt + rho * (tnn * ut + tnt * un));
+  } else if constexpr (N == 6) {
+    return w * (rho * (ttt + s.ut2)) + (0.4 * qn + 2.0 * ut * (rho * tnt));
+  } else if constexpr (N == 7) {
+    const double u = qn + s.E * un + rho * s.An;
+    const double p = un * (2.2 * qn + rho * s.An) + 0.4 * qt * ut
+                   + 0.5 * rho * (tnn * s.vs + th * (7.0 * tnn - 2.0 * th));
+    return w * u + p;
+  } else {
+    const double u = qt + s.E * ut + rho * s.At;
+    const double p = 1.4 * qn * ut + 0.4 * qt * un
+                   + rho * (s.unut * tnn + tnt * (0.5 * s.un2 + 1.5 * s.ut2 + 3.5 * th));
+    return w * u + p;
+  }
 }
 
 /* HLL face flux (Flux.c:332-379), frame-local. Fs = (dt/dx) * flux.
- * smax = |(un_L+un_R)/2| + max(c_L, c_R), c = 2 sqrt(5 theta / 3). */
+ * smax = |(un_L+un_R)/2| + max(c_L, c_R), c = 2 sqrt(5 theta / 3).
+ * With a = lamR - lamL > 0 the reference's
+ *   (lamR F_L - lamL F_R + lamL lamR (U_R - U_L)) / a
+ * equals (lamR g_L - lamL g_R) / a for the shifted fluxes g = F - lam U of
+ * each side (gshift): two products per component instead of four. With
+ * a <= 0 (lamL = lamR = 0) both sides' g are F and the flux is their mean. */
 MBM_DEV void hll(const double *WL, const double *WR, double dtdx, double *Fs,
                  double &smax)
 {
@@ -299,27 +335,26 @@ MBM_DEV void hll(const double *WL, const double *WR, double dtdx, double *Fs,
   const double lamL = fmin(WL[1] - cL, 0.0);
   const double lamR = fmax(WR[1] + cR, 0.0);
   const double alph = lamR - lamL;
-  double ca, cb, cc;
+  double ca, cb;
   if (alph > 0.0) {
     const double s = dtdx * rcp(alph);
     ca = lamR * s;
     cb = -lamL * s;
-    cc = lamL * lamR * s;
   } else {
     ca = 0.5 * dtdx;
     cb = 0.5 * dtdx;
-    cc = 0.0;
   }
+  const double wL = WL[1] - lamL, wR = WR[1] - lamR;
   const Side sL = side_of(WL), sR = side_of(WR);
-  hll_comp<0>(WL, sL, WR, sR, ca, cb, cc, Fs);
-  hll_comp<1>(WL, sL, WR, sR, ca, cb, cc, Fs);
-  hll_comp<2>(WL, sL, WR, sR, ca, cb, cc, Fs);
-  hll_comp<3>(WL, sL, WR, sR, ca, cb, cc, Fs);
-  hll_comp<4>(WL, sL, WR, sR, ca, cb, cc, Fs);
-  hll_comp<5>(WL, sL, WR, sR, ca, cb, cc, Fs);
-  hll_comp<6>(WL, sL, WR, sR, ca, cb, cc, Fs);
-  hll_comp<7>(WL, sL, WR, sR, ca, cb, cc, Fs);
-  hll_comp<8>(WL, sL, WR, sR, ca, cb, cc, Fs);
+  Fs[0] = ca * gshift<0>(WL, sL, wL) + cb * gshift<0>(WR, sR, wR);
+  Fs[1] = ca * gshift<1>(WL, sL, wL) + cb * gshift<1>(WR, sR, wR);
+  Fs[2] = ca * gshift<2>(WL, sL, wL) + cb * gshift<2>(WR, sR, wR);
+  Fs[3] = ca * gshift<3>(WL, sL, wL) + cb * gshift<3>(WR, sR, wR);
+  Fs[4] = ca * gshift<4>(WL, sL, wL) + cb * gshift<4>(WR, sR, wR);
+  Fs[5] = ca * gshift<5>(WL, sL, wL) + cb * gshift<5>(WR, sR, wR);
+  Fs[6] = ca * gshift<6>(WL, sL, wL) + cb * gshift<6>(WR, sR, wR);
+  Fs[7] = ca * gshift<7>(WL, sL, wL) + cb * gshift<7>(WR, sR, wR);
+  Fs[8] = ca * gshift<8>(WL, sL, wL) + cb * gshift<8>(WR, sR, wR);
 }
 
 /* x-face: frame = Cartesian. y-face: swap (1,2), (4,6), (7,8) in and out
