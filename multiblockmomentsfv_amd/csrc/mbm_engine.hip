@@ -969,20 +969,22 @@ std::vector<GhostDev> build_ghost_map(const std::vector<mbm_block_desc> &D,
 }
 
 /* Tiles of every block in the order block -> row range -> strip, so that
- * consecutive tiles are horizontally adjacent strips of the same rows. */
+ * consecutive tiles are horizontally adjacent strips of the same rows. Each
+ * block splits its rows evenly (tile_band): no short last tile. */
 std::vector<TileDev> build_tiles(const std::vector<mbm_block_desc> &D, int rows)
 {
   std::vector<TileDev> t;
   for (int k = 0; k < (int)D.size(); k++) {
     const int lx = D[k].nx + 2 * MBC, ly = D[k].ny + 2 * MBC;
-    for (int r0 = 1; r0 <= ly - 3; r0 += rows) {
+    const int band = tile_band(ly, rows);
+    for (int r0 = 1; r0 <= ly - 3; r0 += band) {
       /* strip j: lanes [60j, 60j+63], x-faces [60j+1, 60j+61]; need 60j+61 >= lx-3 */
       for (int a0 = 0;; a0 += STRIP) {
         TileDev d;
         d.blk = k;
         d.a0 = a0;
         d.r0 = r0;
-        d.r1 = std::min(r0 + rows - 1, ly - 3);
+        d.r1 = std::min(r0 + band - 1, ly - 3);
         t.push_back(d);
         if (a0 + 61 >= lx - 3) break;
       }
@@ -996,7 +998,9 @@ std::vector<TileDev> build_tiles(const std::vector<mbm_block_desc> &D, int rows)
  * columns), and workgroup w goes to XCD w mod 8. The groups are dealt so
  * that each XCD receives one contiguous run of groups -- a patch of
  * neighbouring strips and row ranges -- keeping halo and edge lines in that
- * XCD's L2. Padding entries (blk = -1) exit at once. */
+ * XCD's L2. (Ordering each XCD's run tallest tile first was measured slower
+ * on the 256-deck sweep: 9.91 against 10.03 G cell-updates/s, the patches
+ * lose their locality.) Padding entries (blk = -1) exit at once. */
 std::vector<TileDev> xcd_order(const std::vector<TileDev> &t, int w)
 {
   constexpr int NXCD = 8;
@@ -1021,7 +1025,8 @@ std::vector<TileDev> xcd_order(const std::vector<TileDev> &t, int w)
  * runs w = ceil(tiles / SIMDs) waves over the launch. With two waves resident
  * per SIMD a row costs 0.87 of its one-wave time (the other wave covers the
  * waits; measured on 4x512^2: 78.5 us at 10 rows and two waves per SIMD,
- * 74.1 us at 19 rows and one). Minimises w * (rows + 4) * that factor. */
+ * 74.1 us at 19 rows and one). Minimises w * (tallest tile + 4) * that
+ * factor, with each block's rows split evenly (tile_band). */
 int auto_tile_rows(const std::vector<mbm_block_desc> &D, int simds)
 {
   int best = 32;
@@ -1030,17 +1035,19 @@ int auto_tile_rows(const std::vector<mbm_block_desc> &D, int simds)
   for (const auto &b : D) maxrows = std::max(maxrows, b.ny + 3);
   for (int R = 4; R <= 512; R++) {
     long long nt = 0;
+    int tallest = 0;
     for (const auto &b : D) {
-      const int lx = b.nx + 2 * MBC;
+      const int lx = b.nx + 2 * MBC, band = tile_band(b.ny + 2 * MBC, R);
       int strips = 0;
       for (int a0 = 0;; a0 += STRIP) {
         strips++;
         if (a0 + 61 >= lx - 3) break;
       }
-      nt += (long long)strips * ((b.ny + 3 + R - 1) / R);
+      nt += (long long)strips * ((b.ny + 3 + band - 1) / band);
+      tallest = std::max(tallest, band);
     }
     const long long w = (nt + simds - 1) / simds;
-    const double cost = (double)w * (std::min(R, maxrows) + 4) *
+    const double cost = (double)w * (tallest + 4) *
                         (w >= 2 && MBM_STAGE_WAVES >= 2 ? 0.87 : 1.0);
     if (cost < bestc - 1e-9) {
       bestc = cost;
@@ -1051,7 +1058,7 @@ int auto_tile_rows(const std::vector<mbm_block_desc> &D, int simds)
   return best;
 }
 
-int relayout(mbm_ctx *c, int band);
+int relayout(mbm_ctx *c, int rows);
 
 int upload_tiles(mbm_ctx *c)
 {
@@ -1084,7 +1091,9 @@ int upload_tiles(mbm_ctx *c)
   }
   /* the layers follow the tile height, so that the rows marched together
    * stay adjacent */
-  if (std::max(rows, 2) != c->hblk[0].band) {
+  bool moved = false;
+  for (const BlockDev &B : c->hblk) moved = moved || B.band != tile_band(B.ly, rows);
+  if (moved) {
     const int rc = relayout(c, rows);
     if (rc) {
       (void)hipFree(nt);
@@ -1188,14 +1197,15 @@ int upload_params(mbm_ctx *c)
 
 namespace mbmi {
 
-/* Row order and offsets of the blocks of a layer for tiles of height band
- * (blocks 256-byte aligned); off returns the layer's size in elements. */
-int lay_out(std::vector<BlockDev> &hb, int band, size_t &off)
+/* Row order and offsets of the blocks of a layer for tiles of height rows
+ * (each block's own tile_band; blocks 256-byte aligned); off returns the
+ * layer's size in elements. */
+int lay_out(std::vector<BlockDev> &hb, int rows, size_t &off)
 {
-  band = std::max(band, 2);
   off = 0;
   for (size_t k = 0; k < hb.size(); k++) {
     BlockDev &b = hb[k];
+    const int band = tile_band(b.ly, rows);
     if ((long long)(b.ly - 1) * band >= (1ll << 32))
       return set_err(MBM_E_ARG, "block %zu: %d rows too many for row bands of %d", k + 1, b.ly, band);
     set_block_layout(b, band);
@@ -1228,13 +1238,13 @@ void fill_records(const mbm_ctx *c, std::vector<FillRec> &fr, std::vector<double
   }
 }
 
-/* Moves the three layers to the row order of tile height band; the ghost
+/* Moves the three layers to the row order of tile height rows; the ghost
  * records follow, captured graphs and adjoint buffers are dropped. */
-int relayout(mbm_ctx *c, int band)
+int relayout(mbm_ctx *c, int rows)
 {
   std::vector<BlockDev> nh = c->hblk;
   size_t off = 0;
-  int rc = lay_out(nh, band, off);
+  int rc = lay_out(nh, rows, off);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   if (c->stream2) HIPCHK(hipStreamSynchronize(c->stream2));

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdarg>
 #include <vector>
+#include <algorithm>
 
 #include "mbmfv.h"
 #include "mbmfv_host.h"
@@ -86,6 +87,18 @@ inline void set_block_layout(BlockDev &B, int band)
   B.nbands = (B.ly - 1 + band - 1) / band;
   B.bmul = (unsigned)((0x100000000ull + (unsigned long long)band - 1) / (unsigned long long)band);
   B.npos = band * B.nbands + 1;
+}
+
+/* Tile height of a block with ly rows (ghosts included) for a requested
+ * height: its iteration rows 1..ly-3 split into k = ceil((ly-3)/rows) tiles
+ * of equal height (the last one at most k-1 rows shorter), so that no block
+ * ends in a short tile that costs a full warm-up; at least 2, the smallest
+ * band the row order allows. The block's row order uses the same height. */
+inline int tile_band(int ly, int rows)
+{
+  const int n = ly - 3;
+  const int k = (n + std::max(rows, 1) - 1) / std::max(rows, 1);
+  return std::max((n + k - 1) / k, 2);
 }
 
 /* Device-resident state of the Steps controller of one deck (Main.c:189-244). */
