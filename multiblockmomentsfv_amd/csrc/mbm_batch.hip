@@ -139,8 +139,10 @@ extern "C" int mbm_batch_ndecks(const mbm_ctx *c) { return c ? c->ndecks : 0; }
 
 /* Times 'reps' stage pairs (with their fills) of every deck, at the decks'
  * current dt, for each pair of a candidate wave group and tile height, and
- * keeps the fastest pair. The state is not changed: the stages write TMP and
- * the NEW side only. */
+ * keeps the fastest pair. Two passes over the candidates; each candidate
+ * counts with the faster of its two timings, so that a change of the GPU's
+ * clock during one pass does not decide the choice. The state is not
+ * changed: the stages write TMP and the NEW side only. */
 extern "C" int mbm_batch_autotune_shape(mbm_ctx *c, const int *rows, int nrows, const int *waves,
                                         int nwaves, int reps, int *best_rows, int *best_waves,
                                         double *best_ms)
@@ -164,25 +166,31 @@ extern "C" int mbm_batch_autotune_shape(mbm_ctx *c, const int *rows, int nrows, 
                         c->stream));
   int bestr = c->tile_rows, bestw = c->wpg, rc;
   double bestt = 1e300;
-  for (int k = 0; k < nwaves; k++) {
-    if ((rc = mbm_set_wave_group(c, waves[k]))) return rc;
-    for (int i = 0; i < nrows; i++) {
-      if ((rc = mbm_set_tile_rows(c, rows[i]))) return rc;
-      if ((rc = enqueue_device_stages(c))) return rc;    /* warm-up */
-      HIPCHK(hipEventRecord(c->ev[0], c->stream));
-      for (int r = 0; r < reps; r++)
-        if ((rc = enqueue_device_stages(c))) return rc;
-      HIPCHK(hipEventRecord(c->ev[1], c->stream));
-      HIPCHK(hipEventSynchronize(c->ev[1]));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-      if (ms / reps < bestt) {
-        bestt = ms / reps;
+  std::vector<double> t((size_t)nwaves * nrows, 1e300);
+  for (int pass = 0; pass < 2; pass++)
+    for (int k = 0; k < nwaves; k++) {
+      if ((rc = mbm_set_wave_group(c, waves[k]))) return rc;
+      for (int i = 0; i < nrows; i++) {
+        if ((rc = mbm_set_tile_rows(c, rows[i]))) return rc;
+        if ((rc = enqueue_device_stages(c))) return rc;    /* warm-up */
+        HIPCHK(hipEventRecord(c->ev[0], c->stream));
+        for (int r = 0; r < reps; r++)
+          if ((rc = enqueue_device_stages(c))) return rc;
+        HIPCHK(hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        double &tk = t[(size_t)k * nrows + i];
+        tk = std::min(tk, (double)ms / reps);
+      }
+    }
+  for (int k = 0; k < nwaves; k++)
+    for (int i = 0; i < nrows; i++)
+      if (t[(size_t)k * nrows + i] < bestt) {
+        bestt = t[(size_t)k * nrows + i];
         bestr = rows[i];
         bestw = waves[k];
       }
-    }
-  }
   if ((rc = clear_slots(c))) return rc;
   if ((rc = mbm_set_wave_group(c, bestw))) return rc;
   if ((rc = mbm_set_tile_rows(c, bestr))) return rc;
