@@ -1456,12 +1456,32 @@ int enqueue_device_step(mbm_ctx *c, double *buf0, double *buf1, double *tmp)
 
 /* stages and fills of every deck at its current dt, without the controller
  * (writes TMP and NEW only; used to time tile shapes). The fills belong to
- * the timed work, and without them stage 2 would read stale TMP ghosts. */
+ * the timed work, and without them stage 2 would read stale TMP ghosts. A
+ * two-group batch runs its groups on their two streams, as its attempts do;
+ * group 1's stage 2 then reads the TMP of its stage 1 in the previous call. */
 int enqueue_device_stages(mbm_ctx *c)
 {
   double *buf0 = c->layer[c->role[MBM_L_DATA]], *buf1 = c->layer[c->role[MBM_L_NEW]];
   double *tmp = c->layer[c->role[MBM_L_TMP]];
   int rc;
+  if (c->ngroups == 2) {
+    /* the two groups on their two streams, as a batched attempt runs them
+     * (group 1 half a step behind: its stage 2, then its stage 1); the
+     * context's stream waits for both */
+    HIPCHK(hipEventRecord(c->grp_ev, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->stream2, c->grp_ev, 0));
+    if ((rc = enqueue_group_stage(c, 0, 1, buf0, buf1, tmp))) return rc;
+    hipStream_t keep = c->stream;
+    c->stream = c->stream2;
+    rc = enqueue_group_stage(c, 1, 2, buf0, buf1, tmp);
+    if (!rc) rc = enqueue_group_stage(c, 1, 1, buf0, buf1, tmp);
+    c->stream = keep;
+    if (rc) return rc;
+    if ((rc = enqueue_group_stage(c, 0, 2, buf0, buf1, tmp))) return rc;
+    HIPCHK(hipEventRecord(c->grp_ev, c->stream2));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->grp_ev, 0));
+    return MBM_OK;
+  }
   if ((rc = enqueue_group_stage(c, -1, 1, buf0, buf1, tmp))) return rc;
   return enqueue_group_stage(c, -1, 2, buf0, buf1, tmp);
 }
