@@ -641,21 +641,55 @@ __device__ double prod_after_nan(const double *layer, const BlockDev &B, unsigne
   return wave_max(m);
 }
 
-/* CFL of one block and stage from its slots (Main.c:395), the production
- * part refolded after a NaN; clears the slots. All 64 lanes call; the
- * result is uniform. */
+/* CFL of blocks [k0, k1) in one stage from their slots (slots of block k at
+ * SLOTS_PER_BLOCK * k): per block dt*cx/dx + dt*cy/dy + cprod (Main.c:395),
+ * the production part refolded after a NaN; over the blocks the running
+ * maximum of Main.c:268-282, which starts at 0 and skips a NaN block, i.e.
+ * the largest non-NaN block CFL or 0 (the order does not matter). One lane
+ * per block reads and clears its slots; a block with a NaN production CFL
+ * is refolded by the whole wavefront. All 64 lanes call; the result is
+ * uniform. */
 template <int STAGE>
-__device__ double block_cfl(unsigned long long *s, const BlockDev &B, const double *in, double dt,
-                            double dtkn, double force)
+__device__ double stage_cfl(unsigned long long *slots, const BlockDev *blocks, int k0, int k1,
+                            const double *in, double dt, double dtkn, double force)
 {
-  const double cx = __longlong_as_double((long long)s[0]);
-  const double cy = __longlong_as_double((long long)s[1]);
-  double cprod = __longlong_as_double((long long)s[2]);
-  const unsigned long long pnan = s[3];
-  if (pnan) cprod = prod_after_nan<STAGE>(in, B, pnan, dt, dtkn, force);
-  __syncthreads();
-  if (threadIdx.x == 0) { s[0] = 0ull; s[1] = 0ull; s[2] = 0ull; s[3] = 0ull; }
-  return dt * cx / B.dx + dt * cy / B.dy + cprod;
+  const int lane = threadIdx.x & 63;
+  double c = 0.0;
+  for (int base = k0; base < k1; base += 64) {
+    const int k = base + lane;
+    const bool have = k < k1;
+    unsigned long long s0 = 0ull, s1 = 0ull, s2 = 0ull, s3 = 0ull;
+    if (have) {
+      unsigned long long *sk = slots + (size_t)SLOTS_PER_BLOCK * k;
+      s0 = sk[0];
+      s1 = sk[1];
+      s2 = sk[2];
+      s3 = sk[3];
+      sk[0] = 0ull;
+      sk[1] = 0ull;
+      sk[2] = 0ull;
+      sk[3] = 0ull;
+    }
+    double cprod = __longlong_as_double((long long)s2);
+    unsigned long long nan_lanes = __builtin_amdgcn_ballot_w64(have && s3 != 0ull);
+    while (nan_lanes) {
+      const int l = __builtin_ctzll(nan_lanes);
+      nan_lanes &= nan_lanes - 1ull;
+      const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(s3 & 0xffffffffull), l);
+      const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(s3 >> 32), l);
+      const double r = prod_after_nan<STAGE>(in, blocks[base + l],
+                                             ((unsigned long long)hi << 32) | lo, dt, dtkn, force);
+      if (lane == l) cprod = r;
+    }
+    if (have) {
+      const BlockDev &B = blocks[k];
+      const double cx = __longlong_as_double((long long)s0);
+      const double cy = __longlong_as_double((long long)s1);
+      const double v = dt * cx / B.dx + dt * cy / B.dy + cprod;
+      if (v > c) c = v;
+    }
+  }
+  return wave_max(c);
 }
 
 /* Steps controller of every deck (Main.c:202-216) after one TimeUpdate:
@@ -673,16 +707,9 @@ __global__ void __launch_bounds__(64) control_kernel(DeckState *decks, unsigned 
   if (S.done) return;
   const double dtkn = S.dt / S.kn;
   const double *data = S.parity ? buf1 : buf0;
-  double c1 = 0.0, c2 = 0.0;
-  for (int k = first[d]; k < first[d + 1]; k++) {
-    const double v = block_cfl<1>(slots + (size_t)SLOTS_PER_BLOCK * k, blocks[k], data, S.dt, dtkn, S.force);
-    if (v > c1) c1 = v;
-  }
-  for (int k = first[d]; k < first[d + 1]; k++) {
-    const double v = block_cfl<2>(slots + (size_t)SLOTS_PER_BLOCK * (nb + k), blocks[k], tmp, S.dt,
-                                  dtkn, S.force);
-    if (v > c2) c2 = v;
-  }
+  const double c1 = stage_cfl<1>(slots, blocks, first[d], first[d + 1], data, S.dt, dtkn, S.force);
+  const double c2 = stage_cfl<2>(slots + (size_t)SLOTS_PER_BLOCK * nb, blocks, first[d], first[d + 1],
+                                 tmp, S.dt, dtkn, S.force);
   if (threadIdx.x != 0) return;
   const double m2 = c2 > 0.0 ? c2 : 0.0;
   const double cfl = c1 > m2 ? c1 : m2;
@@ -720,16 +747,8 @@ __global__ void __launch_bounds__(64) finalize_kernel(unsigned long long *slots,
                                                       int nb, double dt, double kn, double force,
                                                       const double *in1, const double *in2, double *out)
 {
-  double c1 = 0.0, c2 = 0.0;
-  for (int k = 0; k < nb; k++) {
-    const double v = block_cfl<1>(slots + (size_t)SLOTS_PER_BLOCK * k, blocks[k], in1, dt, dt / kn, force);
-    if (v > c1) c1 = v;
-  }
-  for (int k = 0; k < nb; k++) {
-    const double v = block_cfl<2>(slots + (size_t)SLOTS_PER_BLOCK * (nb + k), blocks[k], in2, dt, dt / kn,
-                                  force);
-    if (v > c2) c2 = v;
-  }
+  const double c1 = stage_cfl<1>(slots, blocks, 0, nb, in1, dt, dt / kn, force);
+  const double c2 = stage_cfl<2>(slots + (size_t)SLOTS_PER_BLOCK * nb, blocks, 0, nb, in2, dt, dt / kn, force);
   if (threadIdx.x != 0) return;
   const double m2 = c2 > 0.0 ? c2 : 0.0;
   out[0] = c1 > m2 ? c1 : m2;
