@@ -102,11 +102,8 @@ extern "C" int mbm_batch_run(mbm_ctx *c, const double *tend, const double *cflre
                         c->stream));
   /* roles stay fixed in a batch: each deck's parity says where its DATA is */
   const int ndecks = c->ndecks;
-  int rc = device_steps(c, 64, st.data());
+  int rc = device_steps(c, 64, st.data(), st.data());
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(st.data(), c->ddecks, st.size() * sizeof(DeckState), hipMemcpyDeviceToHost,
-                        c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
   for (int d = 0; d < ndecks; d++) {
     t[d] = st[d].t;
     if (nsteps) nsteps[d] = st[d].nstep;

@@ -1594,7 +1594,7 @@ int group1_graph(mbm_ctx *c, double *buf0, double *buf1, double *tmp, bool lng)
   return MBM_OK;
 }
 
-int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known)
+int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known, DeckState *last)
 {
   const int iD = c->role[MBM_L_DATA], iN = c->role[MBM_L_NEW], iT = c->role[MBM_L_TMP];
   double *buf0 = c->layer[iD], *buf1 = c->layer[iN], *tmp = c->layer[iT];
@@ -1654,6 +1654,7 @@ int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known)
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   if (!c->batch && st[0].parity) std::swap(c->role[MBM_L_DATA], c->role[MBM_L_NEW]);
+  if (last) std::copy(st.begin(), st.end(), last);
   return MBM_OK;
 }
 
@@ -1964,10 +1965,9 @@ extern "C" int mbm_steps(mbm_ctx *c, double *t, double tend, double cflmax, doub
   S.done = !(*t < tend);
   S.reccap = rec ? reccap : 0;
   HIPCHK(hipMemcpyAsync(c->ddecks, &S, sizeof S, hipMemcpyHostToDevice, c->stream));
-  int rc = device_steps(c, 64, &S);
+  /* S: the deck state the loop read back last (after the last attempt) */
+  int rc = device_steps(c, 64, &S, &S);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(&S, c->ddecks, sizeof S, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
   if (rec && S.nrec > 0) {
     std::vector<RecordDev> r(std::min(S.nrec, reccap));
     HIPCHK(hipMemcpy(r.data(), c->drec, r.size() * sizeof(RecordDev), hipMemcpyDeviceToHost));

@@ -235,8 +235,9 @@ int enqueue_stage(mbm_ctx *c, int stage, const double *in, const double *in0, do
                   double dt, unsigned long long *slots);
 mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_first, int ndecks,
                     const mbm_params *deck_par, void *stream, int *rc);
-/* known: the deck states just written to the device (NULL: read them back) */
-int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known);
+/* known: the deck states just written to the device (NULL: read them back);
+ * last (may be NULL, may be known): receives the deck states after the run */
+int device_steps(mbm_ctx *c, int maxchunk, const DeckState *known, DeckState *last);
 int enqueue_time_update(mbm_ctx *c, double dt);   /* both stages and fills, CFL to device */
 int upload_tiles(mbm_ctx *c);
 int validate(const mbm_block_desc *blocks, int nb, const mbm_params *p);
