@@ -414,11 +414,33 @@ MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
     bad = !finite_relaxed(P);
     if constexpr (NEED_INIT) {
       const Damp d0 = damp0(z, i1, i2);
-      double R[9];
-      relax(Q, d0, R);
-      R[1] += dt * force;
-      conservative(R, I);
       cp = 0.5 * (1.0 - d0.d2);
+      if (force == 0.0 && __builtin_expect(!bad, 1)) {
+        /* Conservative(relax(Primitive(U))) taken from U itself: relaxing
+         * removes rho (1-d1) (T_ij - theta delta_ij) from the second
+         * moments, and (1-d2) q plus the work of that removed stress
+         * against v from the heat fluxes; density, momentum and energy are
+         * unchanged */
+        const double rom = Q[0] * (1.0 - d0.d1), om2 = 1.0 - d0.d2;
+        const double a11 = Q[4] - Q[3], a22 = Q[6] - Q[3];
+        I[0] = U[0];
+        I[1] = U[1];
+        I[2] = U[2];
+        I[3] = U[3];
+        I[4] = U[4] - rom * a11;
+        I[5] = U[5] - rom * Q[5];
+        I[6] = U[6] - rom * a22;
+        I[7] = U[7] - om2 * Q[7] - rom * (a11 * Q[1] + Q[5] * Q[2]);
+        I[8] = U[8] - om2 * Q[8] - rom * (Q[5] * Q[1] + a22 * Q[2]);
+      } else {
+        /* a body force (Flux.c:239-253 adds dt*force to v1 before the
+         * round trip), or non-finite data, whose NaNs follow the
+         * reference's round trip */
+        double R[9];
+        relax(Q, d0, R);
+        R[1] += dt * force;
+        conservative(R, I);
+      }
     }
   } else {
     Damp d3;
