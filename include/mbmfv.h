@@ -186,16 +186,22 @@ int mbm_layer_set(mbm_ctx *c, int layer, const void *src, long long n);
  * No counterpart in the reference (it has no derivatives). Reverse-mode
  * derivative of a run of TimeUpdate steps with the dt schedule held fixed,
  * with respect to the state and Kn; limiter, |.| and HLL speed clipping are
- * differentiated almost everywhere. Memory: sqrt(N) checkpointing, N/stride
- * + 2*stride - 1 layers, allocated on first use and kept (for the next
- * forward/backward pair) until mbm_adj_release or mbm_destroy. A backward
- * pass leaves the recorded run intact, so it may be repeated.
+ * differentiated almost everywhere. Layers are allocated on first use and
+ * kept (for the next forward/backward pair) until mbm_adj_release or
+ * mbm_destroy. A backward pass leaves the recorded run intact, so it may be
+ * repeated.
  *
  * mbm_adj_forward: advances DATA from its current value. dts == NULL: the
  * Steps controller of Main.c:189-244 (from DT_INITIAL, rejection, clamp to
  * tend, at most maxsteps accepted steps) and its accepted dt are recorded;
- * otherwise the ndt given steps are replayed. DATA is checkpointed every
- * 'stride' steps. */
+ * otherwise the ndt given steps are replayed. stride >= 1: DATA is
+ * checkpointed every 'stride' steps and the backward pass recomputes each
+ * segment (N/stride + 2*(2*stride - 1) layers). stride == 0: a replay whose
+ * N + 2 layers fit the budget keeps every step's DATA state and, as far as
+ * the budget goes, the stage states of the last steps, so the backward pass
+ * recomputes at most stage 1 of a step; otherwise stride ceil(sqrt(N))
+ * (N = ndt, or maxsteps, or 1000 without either). The gradients do not
+ * depend on the mode. */
 int mbm_adj_forward(mbm_ctx *c, double t0, double tend, double cflmax, double cflreg,
                     int maxsteps, const double *dts, int ndt, int stride,
                     int *nsteps, double *t_end);
@@ -209,6 +215,14 @@ int mbm_adj_schedule(mbm_ctx *c, double *dts, int cap, int *n);
 int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial, long long n,
                      double *g_kn);
 int mbm_adj_release(mbm_ctx *c);
+/* device memory that a stride-0 recording may hold for its states, in
+ * bytes; 0 (the default) = the free device memory plus the layers the
+ * recording already holds, less max(8 GiB, 5 % of the device) */
+int mbm_adj_set_budget(mbm_ctx *c, long long bytes);
+/* device memory held by the recorded states (checkpoints, kept stage
+ * states, the reverse sweep's segment layers, and layers kept for reuse by
+ * the next recording) */
+int mbm_adj_memory(mbm_ctx *c, long long *bytes);
 
 /* ---- batches of independent decks ---------------------------------------
  * No counterpart in the reference (one deck per process there). Decks are

@@ -129,6 +129,8 @@ def lib():
         "mbm_adj_schedule": (i, [p, p, i, ctypes.POINTER(i)]),
         "mbm_adj_backward": (i, [p, p, p, ctypes.c_longlong, ctypes.POINTER(d)]),
         "mbm_adj_release": (i, [p]),
+        "mbm_adj_set_budget": (i, [p, ctypes.c_longlong]),
+        "mbm_adj_memory": (i, [p, ctypes.POINTER(ctypes.c_longlong)]),
         "mbm_batch_create": (i, [ctypes.POINTER(BlockDesc), p, i, ctypes.POINTER(Params), p,
                                  ctypes.POINTER(p)]),
         "mbm_batch_ndecks": (i, [p]),
@@ -173,7 +175,9 @@ def exported_symbols():
                         "mbm_interior_cells", "mbm_block_dims", "mbm_device", "mbm_enable_timing",
                         "mbm_kernel_time", "mbm_layer_elems", "mbm_layer_get", "mbm_layer_set",
                         "mbm_adj_forward", "mbm_adj_schedule", "mbm_adj_backward",
-                        "mbm_adj_release", "mbm_batch_create", "mbm_batch_ndecks",
+                        "mbm_adj_release", "mbm_adj_set_budget", "mbm_adj_memory",
+                        "mbm_batch_create",
+                        "mbm_batch_ndecks",
                         "mbm_batch_run", "mbm_batch_download", "mbm_batch_autotune",
                         "mbm_batch_autotune_shape")]
 
@@ -459,14 +463,17 @@ class Engine:
 
     def adj_forward(self, tend, cflreg, cflmax=1.0, maxsteps=0, dts=None, stride=None, t0=0.0):
         """Advances DATA recording (dts=None) or replaying a dt schedule,
-        with checkpoints every `stride` steps. Returns (nsteps, t_end)."""
+        with checkpoints every `stride` steps. A replay with stride None (or
+        0) keeps the states of every step in device memory when the budget
+        (adj_set_budget) holds them, else checkpoints every ceil(sqrt(N))
+        steps. Returns (nsteps, t_end)."""
         n, te = ctypes.c_int(), ctypes.c_double()
         self.adj_runs += 1
         if dts is not None:
             dts = np.ascontiguousarray(dts, dtype=np.float64)
-            stride = stride or max(1, int(np.ceil(np.sqrt(len(dts)))))
             _check(lib().mbm_adj_forward(self.h, t0, tend, cflmax, cflreg, 0, _ptr(dts), len(dts),
-                                         stride, ctypes.byref(n), ctypes.byref(te)), "mbm_adj_forward")
+                                         int(stride or 0), ctypes.byref(n), ctypes.byref(te)),
+                   "mbm_adj_forward")
         else:
             stride = stride or max(1, int(np.ceil(np.sqrt(maxsteps if maxsteps > 0 else 1000))))
             _check(lib().mbm_adj_forward(self.h, t0, tend, cflmax, cflreg, maxsteps, None, 0, stride,
@@ -492,6 +499,17 @@ class Engine:
                                       self.layer_elems(), ctypes.byref(gk)), "mbm_adj_backward")
         del keep_g, keep_i
         return g_initial, gk.value
+
+    def adj_set_budget(self, nbytes):
+        """Device memory a resident recording may hold (0: the free memory
+        less a reserve)."""
+        _check(lib().mbm_adj_set_budget(self.h, int(nbytes)), "mbm_adj_set_budget")
+
+    def adj_memory(self):
+        """Device bytes held by the recorded states."""
+        b = ctypes.c_longlong()
+        _check(lib().mbm_adj_memory(self.h, ctypes.byref(b)), "mbm_adj_memory")
+        return b.value
 
     def adj_release(self):
         _check(lib().mbm_adj_release(self.h), "mbm_adj_release")

@@ -33,6 +33,15 @@
  * its DATA and stage states (memory: N/stride + 2 x (2*stride - 1) layers,
  * kept between runs until mbm_adj_release). Segment s-1 is recomputed on a
  * second stream while segment s is reversed.
+ * Resident recording (stride 0 with a given schedule, when the memory
+ * budget holds N + 2 layers): the forward run writes the DATA state of every
+ * step, and the stage state of the last steps as far as the budget goes,
+ * into layers of their own (no copies; a step is the same launches as in
+ * the checkpointed run, so the results are bitwise the same). The reverse
+ * sweep then recomputes only stage 1 of the steps whose stage state was not
+ * kept, on the second stream, one step ahead. At 4 x 512^2 (77 MB a layer)
+ * a 2199-step run keeps its 2199 DATA states and about 1200 stage states in
+ * 288 GB.
  */
 #include "mbm_internal.h"
 #include "mbm_dev.h"
@@ -40,6 +49,7 @@
 #include "mbm_adj_stage.h"
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <tuple>
 #include <type_traits>
@@ -73,6 +83,12 @@ struct mbm_adj_state {
   std::vector<double *> ckpt;     /* ckpt[i], i < nckpt: DATA after i*stride steps */
   int nckpt = 0;
   std::vector<double *> seg;      /* segment states during the reverse sweep */
+  /* resident recording: ckpt[k] is the DATA state of step k (stride 1), and
+   * step k >= kept_from has its stage state in tkeep[k - kept_from] */
+  bool resident = false;
+  int kept_from = 0;
+  std::vector<double *> tkeep;
+  std::vector<double *> spare;    /* held layers no pool uses at present */
   double *g[3] = { nullptr, nullptr, nullptr };   /* gN, gD, gT */
   double *gpy = nullptr;          /* y-part of gP of the current stage */
   TileDev *xtiles = nullptr;
@@ -716,17 +732,61 @@ int step_adjoint(mbm_ctx *c, const double *D, const double *T, double dt, double
   return stage_adjoint<1>(c, D, D, A->g[2], gD, nullptr, gN, dt);
 }
 
-/* layer i of a pool, allocated on first use */
+/* layer i of a pool: on first use taken from the spare layers, else
+ * allocated (a failed allocation leaves no error state in the runtime) */
 int pool_layer(mbm_ctx *c, std::vector<double *> &pool, int i, double **p)
 {
+  mbm_adj_state *A = c->adj;
   while ((int)pool.size() <= i) {
     double *q = nullptr;
-    if (hipMalloc(&q, c->layer_elems * sizeof(double)) != hipSuccess)
+    if (!A->spare.empty()) {
+      q = A->spare.back();
+      A->spare.pop_back();
+    } else if (hipMalloc(&q, c->layer_elems * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
       return set_err(MBM_E_NOMEM, "adjoint: allocation of checkpoint layer %d failed", i);
+    }
     pool.push_back(q);
   }
   *p = pool[i];
   return MBM_OK;
+}
+
+/* a pool keeps its first n layers; the rest become spare */
+void trim_pool(mbm_adj_state *A, std::vector<double *> &pool, size_t n)
+{
+  while (pool.size() > n) {
+    A->spare.push_back(pool.back());
+    pool.pop_back();
+  }
+}
+
+/* the layers of a resident recording of N steps keeping the stage states of
+ * the last *nkeep: N DATA states, two layers for recomputed stage states
+ * (unless all are kept), then the kept stage states as far as the
+ * allocations go (*nkeep lowered if they stop early). false when the DATA
+ * states or the two layers cannot be had. */
+bool reserve_resident(mbm_ctx *c, int N, int *nkeep)
+{
+  mbm_adj_state *A = c->adj;
+  trim_pool(A, A->tkeep, (size_t)*nkeep);
+  trim_pool(A, A->ckpt, (size_t)N);
+  trim_pool(A, A->seg, *nkeep < N ? 2 : 0);
+  double *p;
+  for (int i = 0; i < N; i++)
+    if (pool_layer(c, A->ckpt, i, &p)) return false;
+  for (int i = 0; i < 2 && *nkeep < N; i++)
+    if (pool_layer(c, A->seg, i, &p)) return false;
+  for (int i = 0; i < *nkeep; i++) {
+    if (pool_layer(c, A->tkeep, i, &p)) {
+      *nkeep = *nkeep < N ? i : std::max(0, i - 2);
+      trim_pool(A, A->tkeep, (size_t)*nkeep);
+      for (int j = 0; j < 2; j++)
+        if (pool_layer(c, A->seg, j, &p)) return false;
+      break;
+    }
+  }
+  return true;
 }
 
 int push_checkpoint(mbm_ctx *c, const double *src)
@@ -748,6 +808,25 @@ void free_checkpoints(mbm_adj_state *A)
   A->nckpt = 0;
   for (double *p : A->seg) (void)hipFree(p);
   A->seg.clear();
+  for (double *p : A->tkeep) (void)hipFree(p);
+  A->tkeep.clear();
+  for (double *p : A->spare) (void)hipFree(p);
+  A->spare.clear();
+}
+
+/* layers the recording may hold: the set budget, or the free device memory
+ * plus the layers already held less a reserve of max(8 GiB, 5 %) */
+long long budget_layers(mbm_ctx *c)
+{
+  mbm_adj_state *A = c->adj;
+  const long long lb = (long long)(c->layer_elems * sizeof(double));
+  if (c->adj_budget > 0) return c->adj_budget / lb;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+  const long long held =
+      (long long)(A->ckpt.size() + A->seg.size() + A->tkeep.size() + A->spare.size()) * lb;
+  const long long reserve = std::max<long long>(8ll << 30, (long long)(tot / 20));
+  return std::max<long long>(0, ((long long)fr + held - reserve) / lb);
 }
 
 } // namespace
@@ -788,17 +867,59 @@ extern "C" int mbm_adj_forward(mbm_ctx *c, double t0, double tend, double cflmax
                                int maxsteps, const double *dts, int ndt, int stride,
                                int *nsteps, double *t_end)
 {
-  if (!c || stride < 1) return set_err(MBM_E_ARG, "invalid argument");
+  if (!c || stride < 0 || (dts && ndt < 0)) return set_err(MBM_E_ARG, "invalid argument");
   int rc = adj_setup(c);
   if (rc) return rc;
   mbm_adj_state *A = c->adj;
   A->nckpt = 0;
   A->dts.clear();
+  A->resident = false;
+  A->kept_from = 0;
+  if (stride == 0) {
+    const long long avail = dts && ndt > 0 ? budget_layers(c) : 0;
+    if (avail >= (long long)ndt + 2) {
+      /* every DATA state; the stage states of the last steps, all of them
+       * when 2N layers fit, else as many as leave two layers for the stage
+       * states the reverse sweep recomputes */
+      const long long extra = avail - ndt;
+      int nkeep = extra >= ndt ? ndt : (int)(extra - 2);
+      if (reserve_resident(c, ndt, &nkeep)) {
+        A->resident = true;
+        A->kept_from = ndt - nkeep;
+        stride = 1;
+      }
+    }
+    if (!A->resident) {
+      const int n = dts ? ndt : (maxsteps > 0 ? maxsteps : 1000);
+      stride = std::max(1, (int)std::ceil(std::sqrt((double)std::max(n, 1))));
+    }
+  }
+  if (!A->resident) {
+    /* the checkpointed run's layers: N/stride checkpoints (N known for a
+     * replay), 2 x (2 stride - 1) segment layers; the rest become spare */
+    trim_pool(A, A->tkeep, 0);
+    if (dts) trim_pool(A, A->ckpt, (size_t)std::max(1, (ndt + stride - 1) / stride));
+    trim_pool(A, A->seg, (size_t)(2 * (2 * stride - 1)));
+  }
   A->stride = stride;
-  if ((rc = push_checkpoint(c, c->layer[c->role[MBM_L_DATA]]))) return rc;
   double t = t0;
   int k = 0;
-  if (dts) {
+  if (A->resident) {
+    /* step k reads ckpt[k], writes its stage state to tkeep[k - kept_from]
+     * or TMP and its result to ckpt[k + 1], the last one to DATA */
+    HIPCHK(hipMemcpyAsync(A->ckpt[0], c->layer[c->role[MBM_L_DATA]],
+                          c->layer_elems * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    A->nckpt = ndt;
+    for (k = 0; k < ndt; k++) {
+      double *T = k >= A->kept_from ? A->tkeep[k - A->kept_from] : c->layer[c->role[MBM_L_TMP]];
+      double *N = k + 1 < ndt ? A->ckpt[k + 1] : c->layer[c->role[MBM_L_DATA]];
+      if ((rc = step_buffers(c, A->ckpt[k], T, N, dts[k]))) return rc;
+      t += dts[k];
+      A->dts.push_back(dts[k]);
+    }
+  } else if ((rc = push_checkpoint(c, c->layer[c->role[MBM_L_DATA]]))) {
+    return rc;
+  } else if (dts) {
     /* replay: no CFL, the steps queue without host round trips */
     for (k = 0; k < ndt; k++) {
       if ((rc = step_buffers(c, c->layer[c->role[MBM_L_DATA]], c->layer[c->role[MBM_L_TMP]],
@@ -863,10 +984,14 @@ extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial
    * recomputation of segment s-1 (recompute stream, set (s-1)&1) runs while
    * segment s is reversed (context stream, set s&1) */
   const int L = std::min(S, std::max(N, 1));     /* steps in the longest segment */
+  /* resident recording: segments of one step; steps k >= kept_from have
+   * their stage state kept and need no recomputation */
+  const int kept = A->resident ? A->kept_from : N;
   std::vector<double *> sD[2], sT[2];
   for (int q = 0; q < 2; q++) {
     sD[q].assign(L, nullptr);
     sT[q].assign(L, nullptr);
+    if (kept == 0) continue;
     const int base = q * (2 * L - 1);
     for (int i = 0; i < L; i++) {
       if (i > 0 && (rc = pool_layer(c, A->seg, base + i - 1, &sD[q][i]))) return rc;
@@ -882,6 +1007,7 @@ extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial
   auto recompute = [&](int s) -> int {
     const int q = s & 1, k0 = s * S, k1 = std::min(N, k0 + S);
     sD[q][0] = A->ckpt[s];
+    if (k0 >= kept) return MBM_OK;
     /* the stage and fill launchers enqueue on c->stream */
     hipStream_t keep = c->stream;
     c->stream = A->rstream;
@@ -902,9 +1028,10 @@ extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial
       if (s + 1 < nseg) HIPCHK(hipStreamWaitEvent(A->rstream, A->ev_done[(s + 1) & 1], 0));
       if ((rc = recompute(s - 1))) return rc;
     }
-    HIPCHK(hipStreamWaitEvent(c->stream, A->ev_ready[q], 0));
+    if (k0 < kept) HIPCHK(hipStreamWaitEvent(c->stream, A->ev_ready[q], 0));
     for (int k = k1 - 1; k >= k0; k--) {
-      if ((rc = step_adjoint(c, sD[q][k - k0], sT[q][k - k0], A->dts[k], gN, gD))) return rc;
+      const double *T = k >= kept ? A->tkeep[k - kept] : sT[q][k - k0];
+      if ((rc = step_adjoint(c, sD[q][k - k0], T, A->dts[k], gN, gD))) return rc;
       std::swap(gN, gD);
     }
     HIPCHK(hipEventRecord(A->ev_done[q], c->stream));
@@ -918,6 +1045,23 @@ extern "C" int mbm_adj_backward(mbm_ctx *c, const void *g_final, void *g_initial
   double s = 0.0;
   for (double v : part) s += v;
   if (g_kn) *g_kn = s;
+  return MBM_OK;
+}
+
+extern "C" int mbm_adj_set_budget(mbm_ctx *c, long long bytes)
+{
+  if (!c || bytes < 0) return set_err(MBM_E_ARG, "invalid argument");
+  c->adj_budget = bytes;
+  return MBM_OK;
+}
+
+extern "C" int mbm_adj_memory(mbm_ctx *c, long long *bytes)
+{
+  if (!c || !bytes) return set_err(MBM_E_ARG, "NULL argument");
+  const mbm_adj_state *A = c->adj;
+  *bytes = A ? (long long)(A->ckpt.size() + A->seg.size() + A->tkeep.size() + A->spare.size()) *
+                   (long long)(c->layer_elems * sizeof(double))
+             : 0;
   return MBM_OK;
 }
 

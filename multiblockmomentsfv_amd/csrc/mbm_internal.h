@@ -196,6 +196,7 @@ struct mbm_ctx {
   int nstage, nfill;
   int device;
   mbm_adj_state *adj;         /* adjoint buffers, created on demand */
+  long long adj_budget;       /* bytes for recorded adjoint states, 0 = automatic */
   /* decks (one for a plain context): blocks [first[d], first[d+1]) */
   int ndecks;
   std::vector<int> deck_first;

@@ -414,3 +414,47 @@ def test_adjoint_at_the_configured_shape(mbm):
     fdk = (jp - jm) / (2 * hk * prob.kn)
     assert abs(float(kn.grad) - fdk) <= 1e-6 * abs(fdk), (float(kn.grad), fdk)
     prob.engine.close()
+
+
+def test_resident_recording_matches_checkpointing(mbm):
+    """A replay that keeps its states resident (every DATA state, the stage
+    states as far as the memory budget goes) gives the checkpointed run's
+    final state and gradient bit for bit, whichever split the budget forces;
+    a budget below N + 2 layers falls back to sqrt(N) checkpointing. The
+    layers held show which mode ran."""
+    deck = mbm.read_deck(deck_path("small_kn01"))
+    eng = mbm.Engine(deck.blocks, kn=deck.kn, scenario="tempjump")
+    eng.init()
+    U0 = generic_state(mbm, eng, deck, seed=9)
+    eng.layer_set(0, U0)
+    n, _ = eng.adj_forward(tend=deck.tout[0], cflreg=deck.cfl, maxsteps=23, stride=5)
+    assert n == 23
+    dts = eng.adj_schedule()
+    w = np.random.default_rng(6).normal(size=U0.size) * interior_mask(eng)
+    lb = eng.layer_elems() * 8
+    eng.adj_release()
+    UN = run(eng, U0, dts, stride=5)
+    g, gk = eng.adj_backward(w)
+    assert eng.adj_memory() == (5 + 2 * 9) * lb        # 5 checkpoints, 2 x (2*5 - 1) segment layers
+    # budget in layers -> layers held after forward + backward
+    cases = [(0, 23 + 23),                 # automatic: everything kept
+             (23 + 2 + 10, 23 + 10 + 2),   # 10 stage states kept, 13 recomputed in 2 layers
+             (23 + 2, 23 + 2),             # no stage state kept
+             (23 + 1, 5 + 2 * 9)]          # too small: checkpoints every 5 steps
+    for budget, held in cases:
+        eng.adj_release()
+        eng.adj_set_budget(budget * lb)
+        assert np.array_equal(run(eng, U0, dts), UN), budget
+        g2, gk2 = eng.adj_backward(w)
+        assert np.array_equal(g2, g) and gk2 == gk, budget
+        assert eng.adj_memory() == held * lb, (budget, eng.adj_memory() // lb)
+        g3, gk3 = eng.adj_backward(w)                   # the recording stays intact
+        assert np.array_equal(g3, g) and gk3 == gk, budget
+    # changes of mode without a release: the held layers are reused
+    eng.adj_set_budget(0)
+    for stride in (5, None, 2, None):
+        assert np.array_equal(run(eng, U0, dts, stride=stride), UN), stride
+        g2, gk2 = eng.adj_backward(w)
+        assert np.array_equal(g2, g) and gk2 == gk, stride
+    assert eng.adj_memory() == (23 + 23) * lb           # the largest set of layers used
+    eng.close()
