@@ -379,7 +379,7 @@ def test_layer_buffers_are_checked(mbm):
 @pytest.mark.skipif(not HAVE_TORCH, reason="torch with a GPU is required")
 def test_adjoint_at_the_configured_shape(mbm):
     """The adjoint workload itself: 4x512^2 square4 topology, Reg13, Kn 0.01,
-    harmonic, t_end 0.3 (about 2200 steps, sqrt(N) checkpoints), gradient of
+    harmonic, t_end 0.3 (about 2200 steps, resident recording), gradient of
     sum (theta(t_end) - target)^2 w.r.t. theta0 and Kn, against central
     differences along 4 smooth directions (h = 1e-8) and in Kn (relative
     h = 1e-5): relative agreement <= 1e-6."""
