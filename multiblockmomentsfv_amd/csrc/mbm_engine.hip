@@ -96,12 +96,20 @@ int resolve_scratch(int dev)
 #define MBM_STAGE2_DOWN 1
 #endif
 
+/* the device-controlled attempts copy the interior-sourced ghosts at the end
+ * of the tile that stores their sources (MBM_PUSH=1) instead of in the fill
+ * kernels */
+#ifndef MBM_PUSH
+#define MBM_PUSH 1
+#endif
+
 /* the CFL parts of one tile: face-speed and production maxima (NaN
  * ignored), and 1 + the scan position of its last NaN production CFL */
 struct TileOut {
   double mxx, mxy, mxp;
   unsigned long long pnan;
   int blk;
+  double *out;        /* the layer the tile wrote */
 };
 
 /* One pass of a wavefront over its tile: one Heun stage of Update
@@ -537,6 +545,7 @@ __device__ __forceinline__ int tile_pass(const StageArgs &A, int tix, double (*s
   o.mxp = mxp;
   o.pnan = pnan;
   o.blk = T.blk;
+  o.out = out;
   return (anyslow || mzero != 0ull) ? 2 : 1;
 }
 
@@ -579,6 +588,29 @@ stage_kernel(const StageArgs A)
   /* the last NaN production CFL of this wavefront (rare) */
   if (__builtin_expect(__builtin_amdgcn_ballot_w64(pnan != 0ull) != 0ull, 0))
     if (pnan) atomicMax(A.slots + SLOTS_PER_BLOCK * o.blk + 3, pnan);
+
+  /* ---- ghost copies whose sources this tile stored (SetBoundary's
+   * interior-sourced ghosts, Boundary.c:51-132): the source lines were just
+   * written by this wavefront and are read back from its L2; the ghost
+   * cells of the output layer are read by no tile of this launch ---- */
+  if (A.prange) {
+    const int2 pr = A.prange[tix];
+    if (pr.x < pr.y) {
+      /* this wavefront's stores complete before its loads of the same lines */
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+      double *out = o.out;
+      for (int i = pr.x + lane; i < pr.y; i += 64) {
+        const FillRec g = A.push[i];
+        const double *src = out + g.src;
+        double v[NV];
+#pragma unroll
+        for (int m = 0; m < NV; m++) v[m] = src[(long long)m * g.splane];
+        double *dst = out + g.dst;
+#pragma unroll
+        for (int m = 0; m < NV; m++) dst[(long long)m * g.dplane] = ((g.sign >> m) & 1) ? -v[m] : v[m];
+      }
+    }
+  }
 }
 
 /* SetBoundary as a gather over the precomputed ghost map (Boundary.c:173-193).
@@ -1078,6 +1110,7 @@ int auto_tile_rows(const std::vector<mbm_block_desc> &D, int simds)
 }
 
 int relayout(mbm_ctx *c, int rows);
+int build_push(mbm_ctx *c, const std::vector<TileDev> &t);
 
 int upload_tiles(mbm_ctx *c)
 {
@@ -1125,7 +1158,7 @@ int upload_tiles(mbm_ctx *c)
   c->rows_used = rows;
   if (c->ngroups == 2)
     for (int g = 0; g < 3; g++) c->grp_tile[g] = grp[g];
-  return MBM_OK;
+  return build_push(c, t);
 }
 
 /* raises the dynamic LDS limit of every stage kernel instance to the largest
@@ -1255,6 +1288,103 @@ void fill_records(const mbm_ctx *c, std::vector<FillRec> &fr, std::vector<double
     fr[i].sign = g.sign;
     fxy[i] = make_double2(g.x, g.y);
   }
+}
+
+/* The ghost copies of the stage kernel for the launch list t: each ghost
+ * whose source is an interior cell goes to the tile of t that stores that
+ * cell (strip (a - 2) / STRIP, row band (b - 1) / band, as build_tiles lays
+ * them out), ghosts grouped by tile; scenario ghosts form the residual list
+ * of the fill kernel. Without any ghost the copies stay off. */
+int build_push(mbm_ctx *c, const std::vector<TileDev> &t)
+{
+  auto drop = [&]() {
+    for (void *p : { (void *)c->dpush, (void *)c->dresid, (void *)c->dresidxy, (void *)c->dprange })
+      if (p) (void)hipFree(p);
+    c->dpush = c->dresid = nullptr;
+    c->dresidxy = nullptr;
+    c->dprange = nullptr;
+    c->push_on = false;
+  };
+  drop();
+#if MBM_PUSH
+  if (c->nghost == 0) return MBM_OK;
+  /* tile position by (block, row band, strip) */
+  std::vector<int> base(c->nb + 1, 0), nstrip(c->nb);
+  for (int k = 0; k < c->nb; k++) {
+    const BlockDev &B = c->hblk[k];
+    int ns = 1;
+    while (STRIP * (ns - 1) + 61 < B.lx - 3) ns++;
+    nstrip[k] = ns;
+    base[k + 1] = base[k] + ns * ((B.ly - 3 + B.band - 1) / B.band);
+  }
+  std::vector<int> pos(base[c->nb], -1);
+  for (size_t i = 0; i < t.size(); i++) {
+    const TileDev &d = t[i];
+    if (d.blk < 0) continue;
+    const int q = base[d.blk] + ((d.r0 - 1) / c->hblk[d.blk].band) * nstrip[d.blk] + d.a0 / STRIP;
+    if (q < base[d.blk + 1]) pos[q] = (int)i;
+  }
+  const std::vector<GhostDev> gm = build_ghost_map(c->desc, c->hblk);
+  std::vector<int> owner(gm.size(), -1), cnt(t.size() + 1, 0);
+  std::vector<FillRec> resid;
+  std::vector<double2> residxy;
+  int b1 = c->nb;
+  if (c->ngroups == 2) b1 = c->deck_first[c->grp_deck[1]];
+  c->grp_resid[0] = 0;
+  c->grp_resid[1] = 0;
+  auto rec = [&](const GhostDev &g) {
+    const BlockDev &D = c->hblk[g.dblk];
+    FillRec r;
+    r.dst = cell_elem(D, g.db, g.da);
+    r.dplane = (int)D.plane;
+    r.src = g.sblk >= 0 ? cell_elem(c->hblk[g.sblk], g.sb, g.sa) : -1;
+    r.splane = g.sblk >= 0 ? (int)c->hblk[g.sblk].plane : 0;
+    r.deck = D.deck;
+    r.sign = g.sign;
+    return r;
+  };
+  for (size_t i = 0; i < gm.size(); i++) {
+    const GhostDev &g = gm[i];
+    if (g.sblk < 0) {
+      resid.push_back(rec(g));
+      residxy.push_back(make_double2(g.x, g.y));
+      if (g.dblk < b1) c->grp_resid[1] = (int)resid.size();
+      continue;
+    }
+    /* every source is an interior cell stored by exactly one tile; if one
+     * were not, the fill kernel keeps every ghost */
+    const BlockDev &S = c->hblk[g.sblk];
+    if (g.sa < MBC || g.sa > S.lx - 1 - MBC || g.sb < MBC || g.sb > S.ly - 1 - MBC) return MBM_OK;
+    const int p = pos[base[g.sblk] + ((g.sb - 1) / S.band) * nstrip[g.sblk] + (g.sa - 2) / STRIP];
+    if (p < 0 || t[p].blk != g.sblk || g.sa < t[p].a0 + 2 || g.sa > t[p].a0 + 61 || g.sb < t[p].r0 ||
+        g.sb > t[p].r1)
+      return MBM_OK;
+    owner[i] = p;
+    cnt[p + 1]++;
+  }
+  c->grp_resid[2] = (int)resid.size();
+  if (c->ngroups != 2) c->grp_resid[1] = c->grp_resid[2];
+  for (size_t i = 0; i < t.size(); i++) cnt[i + 1] += cnt[i];
+  std::vector<int2> range(t.size());
+  for (size_t i = 0; i < t.size(); i++) range[i] = make_int2(cnt[i], cnt[i + 1]);
+  std::vector<FillRec> push((size_t)cnt[t.size()]);
+  for (size_t i = 0; i < gm.size(); i++)
+    if (owner[i] >= 0) push[cnt[owner[i]]++] = rec(gm[i]);
+  c->nresid = (int)resid.size();
+  auto up = [&](void **d, const void *h, size_t n) {
+    return hipMalloc(d, std::max<size_t>(n, 1)) == hipSuccess &&
+           (n == 0 || hipMemcpy(*d, h, n, hipMemcpyHostToDevice) == hipSuccess);
+  };
+  if (!up((void **)&c->dpush, push.data(), push.size() * sizeof(FillRec)) ||
+      !up((void **)&c->dprange, range.data(), range.size() * sizeof(int2)) ||
+      !up((void **)&c->dresid, resid.data(), resid.size() * sizeof(FillRec)) ||
+      !up((void **)&c->dresidxy, residxy.data(), residxy.size() * sizeof(double2))) {
+    drop();
+    return set_err(MBM_E_NOMEM, "device allocation of the ghost copy lists failed");
+  }
+  c->push_on = true;
+#endif
+  return MBM_OK;
 }
 
 /* Moves the three layers to the row order of tile height rows; the ghost
@@ -1442,12 +1572,25 @@ int enqueue_group_stage(mbm_ctx *c, int g, int stage, double *buf0, double *buf1
   A.buf1 = buf1;
   A.tmp = tmp;
   A.slots = c->dslots + (stage == 1 ? 0 : SLOTS_PER_BLOCK * c->nb);
+  if (c->push_on) {
+    A.prange = c->dprange + t0;
+    A.push = c->dpush;
+  }
   int rc;
   if ((rc = stage == 1 ? launch_stage<1>(c, A) : launch_stage<2>(c, A))) return rc;
-  if (h1 > h0)
+  if (c->push_on) {
+    /* the stage kernel copied the interior-sourced ghosts; the scenario
+     * ghosts remain */
+    const int r0 = g < 0 ? 0 : c->grp_resid[g], r1 = g < 0 ? c->nresid : c->grp_resid[g + 1];
+    if (r1 > r0)
+      hipLaunchKernelGGL(fill_kernel, dim3((r1 - r0 + 255) / 256), dim3(256), 0, c->stream,
+                         (double *)nullptr, c->dresid + r0, c->dresidxy + r0, r1 - r0, c->par.scenario,
+                         c->dtj, (const DeckState *)c->ddecks, buf0, buf1, tmp, stage);
+  } else if (h1 > h0) {
     hipLaunchKernelGGL(fill_kernel, dim3((h1 - h0 + 255) / 256), dim3(256), 0, c->stream,
                        (double *)nullptr, c->dfill + h0, c->dfillxy + h0, h1 - h0, c->par.scenario, c->dtj,
                        (const DeckState *)c->ddecks, buf0, buf1, tmp, stage);
+  }
   HIPCHK(hipGetLastError());
   return MBM_OK;
 }
@@ -1686,6 +1829,8 @@ extern "C" int mbm_destroy(mbm_ctx *c)
   if (c->dtiles) (void)hipFree(c->dtiles);
   if (c->dfill) (void)hipFree(c->dfill);
   if (c->dfillxy) (void)hipFree(c->dfillxy);
+  for (void *p : { (void *)c->dpush, (void *)c->dresid, (void *)c->dresidxy, (void *)c->dprange })
+    if (p) (void)hipFree(p);
   if (c->dslots) (void)hipFree(c->dslots);
   if (c->dcfl) (void)hipFree(c->dcfl);
   if (c->dtj) (void)hipFree(c->dtj);

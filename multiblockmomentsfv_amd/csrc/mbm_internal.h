@@ -154,6 +154,11 @@ struct StageArgs {
   const DeckState *decks;
   const double *buf0, *buf1;
   double *tmp;
+  /* ghost copies at the end of a tile (null: the fill kernel does them):
+   * tile i copies the ghosts push[prange[i].x .. prange[i].y - 1], whose
+   * sources it stores */
+  const int2 *prange;
+  const FillRec *push;
 };
 
 
@@ -217,6 +222,16 @@ struct mbm_ctx {
    * shifted by half (stage 2 of one group overlaps stage 1 of the other). */
   int ngroups;                    /* 0: one group (plain contexts) */
   int grp_deck[3], grp_tile[3], grp_ghost[3];
+  /* ghost copies by the stage kernel in the device-controlled attempts
+   * (push_on): per tile of the launch list its range of dpush (ghosts with
+   * an interior source, grouped by the tile that stores the source); the
+   * scenario ghosts dresid, ranges grp_resid per group, stay with the fill
+   * kernel */
+  bool push_on;
+  FillRec *dpush, *dresid;
+  double2 *dresidxy;
+  int2 *dprange;
+  int nresid, grp_resid[3];
   hipStream_t stream2;
   hipEvent_t grp_ev;
   hipGraphExec_t graph2, graph2k;
