@@ -703,7 +703,7 @@ int stage1_buffers(mbm_ctx *c, const double *D, double *T, double dt)
   mbm_adj_state *A = c->adj;
   int rc;
   if ((rc = enqueue_stage(c, 1, D, D, T, dt, A->slots))) return rc;
-  return fill_buffer(c, T);
+  return fill_after_stage(c, T);
 }
 
 /* forward step between explicit buffers (frozen dt, no controller); the
@@ -714,7 +714,7 @@ int step_buffers(mbm_ctx *c, const double *D, double *T, double *N, double dt)
   int rc;
   if ((rc = stage1_buffers(c, D, T, dt))) return rc;
   if ((rc = enqueue_stage(c, 2, T, D, N, dt, A->slots + SLOTS_PER_BLOCK * c->nb))) return rc;
-  return fill_buffer(c, N);
+  return fill_after_stage(c, N);
 }
 
 /* reverse of one step at DATA = D with stage state T: gN (in, destroyed)

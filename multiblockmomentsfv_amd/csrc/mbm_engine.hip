@@ -1221,10 +1221,34 @@ int fill_buffer(mbm_ctx *c, double *layer)
 
 int do_fill(mbm_ctx *c, int layer) { return fill_buffer(c, c->layer[c->role[layer]]); }
 
+/* the ghost fill that follows a stage launch: with the stage kernel's ghost
+ * copies only the scenario ghosts remain */
+int fill_after_stage(mbm_ctx *c, double *layer)
+{
+  if (!c->push_on) return fill_buffer(c, layer);
+  if (c->nresid == 0) return MBM_OK;
+  hipLaunchKernelGGL(fill_kernel, dim3((c->nresid + 255) / 256), dim3(256), 0, c->stream, layer,
+                     c->dresid, c->dresidxy, c->nresid, c->par.scenario, c->dtj,
+                     (const DeckState *)nullptr, (double *)nullptr, (double *)nullptr,
+                     (double *)nullptr, 0);
+  HIPCHK(hipGetLastError());
+  return MBM_OK;
+}
+
+/* the stage kernel's ghost copies over the whole tile list, when on */
+void set_push(const mbm_ctx *c, StageArgs &A)
+{
+  if (c->push_on) {
+    A.prange = c->dprange;
+    A.push = c->dpush;
+  }
+}
+
 int enqueue_stage(mbm_ctx *c, int stage, const double *in, const double *in0, double *out,
                   double dt, unsigned long long *slots)
 {
   StageArgs A{};
+  set_push(c, A);
   A.blocks = c->dblk;
   A.tiles = c->dtiles;
   A.ntiles = c->ntiles;
@@ -2013,6 +2037,7 @@ namespace mbmi {
 int enqueue_time_update(mbm_ctx *c, double dt)
 {
   StageArgs A{};
+  set_push(c, A);
   A.blocks = c->dblk;
   A.tiles = c->dtiles;
   A.ntiles = c->ntiles;
@@ -2027,7 +2052,7 @@ int enqueue_time_update(mbm_ctx *c, double dt)
   if (c->timing) HIPCHK(hipEventRecord(c->ev[0], c->stream));
   if ((rc = launch_stage<1>(c, A))) return rc;
   if (c->timing) HIPCHK(hipEventRecord(c->ev[1], c->stream));
-  if ((rc = do_fill(c, MBM_L_TMP))) return rc;
+  if ((rc = fill_after_stage(c, c->layer[c->role[MBM_L_TMP]]))) return rc;
   A.in = c->layer[c->role[MBM_L_TMP]];
   A.in0 = c->layer[c->role[MBM_L_DATA]];
   A.out = c->layer[c->role[MBM_L_NEW]];
@@ -2035,7 +2060,7 @@ int enqueue_time_update(mbm_ctx *c, double dt)
   if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->stream));
   if ((rc = launch_stage<2>(c, A))) return rc;
   if (c->timing) HIPCHK(hipEventRecord(c->ev[3], c->stream));
-  if ((rc = do_fill(c, MBM_L_NEW))) return rc;
+  if ((rc = fill_after_stage(c, c->layer[c->role[MBM_L_NEW]]))) return rc;
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, c->stream, c->dslots, c->dblk, c->nb,
                      dt, c->par.kn, c->par.force, c->layer[c->role[MBM_L_DATA]],
                      c->layer[c->role[MBM_L_TMP]], c->dcfl);

@@ -247,6 +247,7 @@ std::vector<GhostDev> build_ghost_map(const std::vector<mbm_block_desc> &D,
                                       const std::vector<BlockDev> &hb);
 int do_fill(mbm_ctx *c, int layer);
 int fill_buffer(mbm_ctx *c, double *layer);
+int fill_after_stage(mbm_ctx *c, double *layer);
 int enqueue_stage(mbm_ctx *c, int stage, const double *in, const double *in0, double *out,
                   double dt, unsigned long long *slots);
 mbm_ctx *create_ctx(const mbm_block_desc *blocks, int nblocks, const int *deck_first, int ndecks,
