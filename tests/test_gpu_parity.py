@@ -250,3 +250,28 @@ def test_nan_production_cfl_restarts_the_maximum(mbm):
         err = rel_linf(eng.download(layer), orc.get(layer))
         assert err.max() <= REL_LINF_TOL, (layer, err)
     eng.close()
+
+
+@pytest.mark.parametrize("name", ["lshape_small", "periodic", "mixed_bc", "ffs_small"])
+def test_tile_ghost_copies_equal_set_boundary(mbm, name):
+    """The stage kernels copy the interior-sourced ghosts at the end of the
+    tiles that store their sources, in the graph-replayed steps and in
+    TimeUpdate; only scenario (Dirichlet) ghosts keep a fill launch. The
+    ghosts they leave equal a stand-alone SetBoundary (Boundary.c:173-193,
+    the fill kernel over the whole ghost map) bit for bit: DATA after the
+    steps, TMP and NEW after a TimeUpdate (both filled at t + dt,
+    Main.c:253-284)."""
+    deck, eng = engine_for(mbm, name)
+    eng.init()
+    t, n, r, _ = eng.steps(0.0, deck.tout[0], deck.cfl, maxsteps=4, reccap=0)
+    assert n == 4
+    U = eng.layer_get(0)
+    eng.set_boundary(0, t)
+    assert U.tobytes() == eng.layer_get(0).tobytes()
+    dt = eng.next_dt
+    eng.time_update(dt, t)
+    for layer in (1, 2):
+        V = eng.layer_get(layer)
+        eng.set_boundary(layer, t + dt)
+        assert V.tobytes() == eng.layer_get(layer).tobytes(), layer
+    eng.close()
