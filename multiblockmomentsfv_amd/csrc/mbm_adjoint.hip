@@ -863,9 +863,9 @@ extern "C" int mbm_layer_set(mbm_ctx *c, int layer, const void *src, long long n
   return MBM_OK;
 }
 
-extern "C" int mbm_adj_forward(mbm_ctx *c, double t0, double tend, double cflmax, double cflreg,
-                               int maxsteps, const double *dts, int ndt, int stride,
-                               int *nsteps, double *t_end)
+static int adj_forward_run(mbm_ctx *c, double t0, double tend, double cflmax, double cflreg,
+                           int maxsteps, const double *dts, int ndt, int stride, int *nsteps,
+                           double *t_end)
 {
   if (!c || stride < 0 || (dts && ndt < 0)) return set_err(MBM_E_ARG, "invalid argument");
   int rc = adj_setup(c);
@@ -956,6 +956,21 @@ extern "C" int mbm_adj_forward(mbm_ctx *c, double t0, double tend, double cflmax
   if (nsteps) *nsteps = (int)A->dts.size();
   if (t_end) *t_end = t;
   return MBM_OK;
+}
+
+/* a forward run that fails leaves no recording behind (a backward pass is
+ * then refused instead of reversing a partial run) */
+extern "C" int mbm_adj_forward(mbm_ctx *c, double t0, double tend, double cflmax, double cflreg,
+                               int maxsteps, const double *dts, int ndt, int stride,
+                               int *nsteps, double *t_end)
+{
+  const int rc = adj_forward_run(c, t0, tend, cflmax, cflreg, maxsteps, dts, ndt, stride, nsteps,
+                                 t_end);
+  if (rc && c && c->adj) {
+    c->adj->nckpt = 0;
+    c->adj->dts.clear();
+  }
+  return rc;
 }
 
 extern "C" int mbm_adj_schedule(mbm_ctx *c, double *dts, int cap, int *n)

@@ -173,6 +173,26 @@ MBM_DEV void relax(const double *P, Damp d, double *R)
   R[8] = d.d2 * P[8];
 }
 
+/* Conservative(relax(P, d)) with P = Primitive(U), taken from U itself
+ * (relaxation without a body force): relaxing removes rho (1-d1)
+ * (T_ij - theta delta_ij) from the second moments, and (1-d2) q plus the
+ * work of that removed stress against v from the heat fluxes; density,
+ * momentum and energy are unchanged. */
+MBM_DEV void relax_cons(const double *U, const double *P, Damp d, double *V)
+{
+  const double rom = P[0] * (1.0 - d.d1), om2 = 1.0 - d.d2;
+  const double a11 = P[4] - P[3], a22 = P[6] - P[3];
+  V[0] = U[0];
+  V[1] = U[1];
+  V[2] = U[2];
+  V[3] = U[3];
+  V[4] = U[4] - rom * a11;
+  V[5] = U[5] - rom * P[5];
+  V[6] = U[6] - rom * a22;
+  V[7] = U[7] - om2 * P[7] - rom * (a11 * P[1] + P[5] * P[2]);
+  V[8] = U[8] - om2 * P[8] - rom * (P[5] * P[1] + a22 * P[2]);
+}
+
 /* ---------------- slope limiters (Limiter.c:26-59) ---------------- */
 template <int LIM>
 MBM_DEV double limit(double d1, double d2)
@@ -416,22 +436,7 @@ MBM_DEV void project(const double *U, const double *U0, bool ring3, double dt,
       const Damp d0 = damp0(z, i1, i2);
       cp = 0.5 * (1.0 - d0.d2);
       if (force == 0.0 && __builtin_expect(!bad, 1)) {
-        /* Conservative(relax(Primitive(U))) taken from U itself: relaxing
-         * removes rho (1-d1) (T_ij - theta delta_ij) from the second
-         * moments, and (1-d2) q plus the work of that removed stress
-         * against v from the heat fluxes; density, momentum and energy are
-         * unchanged */
-        const double rom = Q[0] * (1.0 - d0.d1), om2 = 1.0 - d0.d2;
-        const double a11 = Q[4] - Q[3], a22 = Q[6] - Q[3];
-        I[0] = U[0];
-        I[1] = U[1];
-        I[2] = U[2];
-        I[3] = U[3];
-        I[4] = U[4] - rom * a11;
-        I[5] = U[5] - rom * Q[5];
-        I[6] = U[6] - rom * a22;
-        I[7] = U[7] - om2 * Q[7] - rom * (a11 * Q[1] + Q[5] * Q[2]);
-        I[8] = U[8] - om2 * Q[8] - rom * (Q[5] * Q[1] + a22 * Q[2]);
+        relax_cons(U, Q, d0, I);
       } else {
         /* a body force (Flux.c:239-253 adds dt*force to v1 before the
          * round trip), or non-finite data, whose NaNs follow the

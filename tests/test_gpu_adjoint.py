@@ -458,3 +458,21 @@ def test_resident_recording_matches_checkpointing(mbm):
         assert np.array_equal(g2, g) and gk2 == gk, stride
     assert eng.adj_memory() == (23 + 23) * lb           # the largest set of layers used
     eng.close()
+
+
+def test_failed_forward_leaves_no_recording(mbm):
+    """A forward run that fails (here an invalid stride) discards the
+    recording, so a backward pass is refused instead of reversing a stale or
+    partial run."""
+    deck = mbm.read_deck(deck_path("box_small"))
+    eng = mbm.Engine(deck.blocks, kn=0.01, scenario="tempjump")
+    eng.init()
+    U0 = generic_state(mbm, eng, deck)
+    w = np.random.default_rng(2).normal(size=U0.size) * interior_mask(eng)
+    run(eng, U0, [1e-3] * 4)
+    eng.adj_backward(w)
+    with pytest.raises(mbm.MbmError):
+        eng.adj_forward(tend=math.inf, cflreg=1.0, dts=[1e-3] * 4, stride=-1)
+    with pytest.raises(mbm.MbmError):
+        eng.adj_backward(w)
+    eng.close()

@@ -107,8 +107,9 @@ def test_face_ref_has_the_reference_nan_pattern(pc, direction):
 def test_projection_nan_pattern(pc, stage):
     """The fast projection (primitive, relaxation, initial value) gives the
     reference's NaN components for every subset of NaN inputs and for a
-    negative temperature (stage 2's identity relaxation is the round trip of
-    Flux.c:230-267 when the data are not finite)."""
+    negative temperature, in the stage input and in the step's input (stage
+    2's identity relaxation is the round trip of Flux.c:230-267 when the data
+    are not finite; so is the relaxation of DATA)."""
     O = oracle.lib()
     orc = oracle.Oracle([dict(nx=4, ny=4, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, bc=[0, 0, 0, 0])],
                         kn=0.001)
@@ -130,13 +131,17 @@ def test_projection_nan_pattern(pc, stage):
         O.orc_conservative(ptr(P), ptr(U))
         return U
 
-    for trial in range(3):
-        P0 = generic_prims(rng)
+    for trial in range(4):
+        P0, Q0 = generic_prims(rng), generic_prims(rng)
         if trial == 2:
             P0[3] = -0.3
-        base, base0 = cons(P0), cons(generic_prims(rng))
+        if trial == 3:
+            Q0[3] = -0.3
+        base, base0 = cons(P0), cons(Q0)
         for mask in range(512):
-            for mask0 in ((0,) if stage == 1 else (0, 2, 272, 511)):
+            # stage 2 takes the DATA part off DATA itself unless that is not
+            # finite: every single NaN component of DATA, and some sets
+            for mask0 in ((0,) if stage == 1 else (0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 272, 511)):
                 U, U0 = base.copy(), base0.copy()
                 U[[k for k in range(9) if mask >> k & 1]] = np.nan
                 U0[[k for k in range(9) if mask0 >> k & 1]] = np.nan
