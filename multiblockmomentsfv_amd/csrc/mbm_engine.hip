@@ -1318,7 +1318,8 @@ void fill_records(const mbm_ctx *c, std::vector<FillRec> &fr, std::vector<double
  * whose source is an interior cell goes to the tile of t that stores that
  * cell (strip (a - 2) / STRIP, row band (b - 1) / band, as build_tiles lays
  * them out), ghosts grouped by tile; scenario ghosts form the residual list
- * of the fill kernel. Without any ghost the copies stay off. */
+ * of the fill kernel. Without any ghost, or if the lists cannot be
+ * allocated, the copies stay off and the fill kernels make them. */
 int build_push(mbm_ctx *c, const std::vector<TileDev> &t)
 {
   auto drop = [&]() {
@@ -1403,8 +1404,11 @@ int build_push(mbm_ctx *c, const std::vector<TileDev> &t)
       !up((void **)&c->dprange, range.data(), range.size() * sizeof(int2)) ||
       !up((void **)&c->dresid, resid.data(), resid.size() * sizeof(FillRec)) ||
       !up((void **)&c->dresidxy, residxy.data(), residxy.size() * sizeof(double2))) {
+    /* the fill kernels do the copies instead; the tile list and layout
+     * stay as they are */
+    (void)hipGetLastError();
     drop();
-    return set_err(MBM_E_NOMEM, "device allocation of the ghost copy lists failed");
+    return MBM_OK;
   }
   c->push_on = true;
 #endif
