@@ -96,9 +96,9 @@ int resolve_scratch(int dev)
 #define MBM_STAGE2_DOWN 1
 #endif
 
-/* the device-controlled attempts copy the interior-sourced ghosts at the end
- * of the tile that stores their sources (MBM_PUSH=1) instead of in the fill
- * kernels */
+/* the stage launches copy the interior-sourced ghosts at the end of the tile
+ * that stores their sources (MBM_PUSH=1) instead of leaving them to the
+ * fill kernels */
 #ifndef MBM_PUSH
 #define MBM_PUSH 1
 #endif

@@ -222,11 +222,10 @@ struct mbm_ctx {
    * shifted by half (stage 2 of one group overlaps stage 1 of the other). */
   int ngroups;                    /* 0: one group (plain contexts) */
   int grp_deck[3], grp_tile[3], grp_ghost[3];
-  /* ghost copies by the stage kernel in the device-controlled attempts
-   * (push_on): per tile of the launch list its range of dpush (ghosts with
-   * an interior source, grouped by the tile that stores the source); the
-   * scenario ghosts dresid, ranges grp_resid per group, stay with the fill
-   * kernel */
+  /* ghost copies by the stage kernel (push_on): per tile of the launch
+   * list its range of dpush (ghosts with an interior source, grouped by the
+   * tile that stores the source); the scenario ghosts dresid, ranges
+   * grp_resid per group, stay with the fill kernel */
   bool push_on;
   FillRec *dpush, *dresid;
   double2 *dresidxy;
