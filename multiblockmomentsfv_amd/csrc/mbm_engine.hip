@@ -103,6 +103,10 @@ int resolve_scratch(int dev)
 #define MBM_PUSH 1
 #endif
 
+#ifndef MBM_STAGE2_REVERSE
+#define MBM_STAGE2_REVERSE 1
+#endif
+
 /* the CFL parts of one tile: face-speed and production maxima (NaN
  * ignored), and 1 + the scan position of its last NaN production CFL */
 struct TileOut {
@@ -563,8 +567,13 @@ stage_kernel(const StageArgs A)
   /* the production CFL (stage 1) is folded by the upward march only */
   static_assert(!DOWN || STAGE == 2, "downward march is stage 2 only");
   const int lane = threadIdx.x & 63;
-  const int tix = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  if (tix >= A.ntiles) return;
+  const int raw = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (raw >= A.ntiles) return;
+  /* stage 2 takes the launch list from its end (MBM_STAGE2_REVERSE): a launch
+   * of several rounds of waves then starts on the tiles stage 1 finished
+   * last, whose lines the memory-side cache still holds; the list is padded
+   * to whole rounds of 8 XCDs, so every XCD keeps a contiguous patch */
+  const int tix = (MBM_STAGE2_REVERSE && STAGE == 2) ? A.ntiles - 1 - raw : raw;
   extern __shared__ double stage_lds[];
   double (*sm)[3][NV][64] = reinterpret_cast<double (*)[3][NV][64]>(stage_lds);
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
