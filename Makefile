@@ -39,7 +39,7 @@ PHYSCHK := tests/native/libphyscheck.so
 .PHONY: all oracle physcheck clean
 all: $(LIB) $(BIN) $(PROBE)
 
-$(PROBE): $(PKG)/csrc/probe/fp64_rate.hip
+$(PROBE): $(PKG)/csrc/probe/fp64_rate.hip $(PKG)/csrc/mbm_phys.h
 	$(HIPCC) $(HIPFLAGS) -shared -o $@ $<
 
 physcheck: $(PHYSCHK)

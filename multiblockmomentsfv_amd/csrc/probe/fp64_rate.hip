@@ -12,6 +12,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include "../mbm_phys.h"
+
 namespace {
 
 constexpr int NACC = 16;
@@ -125,4 +127,33 @@ extern "C" int mbm_probe_fp64_rate(int op, int waves_per_simd, int launches, dou
   if (sink) (void)hipFree(sink);
   if (clk) (void)hipFree(clk);
   return rc;
+}
+
+namespace {
+__global__ void math_kernel(int fn, const double *x, double *y, long long n)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = fn == 0 ? mbm::sqrtp(x[i]) : mbm::rcp(x[i]);
+}
+} // namespace
+
+/* The solver's device square root (fn 0, mbm::sqrtp) and reciprocal (fn 1,
+ * mbm::rcp) of n host doubles x into host y, for accuracy tests. Returns 0
+ * on success. */
+extern "C" int mbm_probe_math(int fn, const double *x, double *y, long long n)
+{
+  if ((fn != 0 && fn != 1) || !x || !y || n < 1 || n > (1ll << 28)) return 1;
+  double *dx = nullptr, *dy = nullptr;
+  const size_t bytes = (size_t)n * sizeof(double);
+  bool ok = hipMalloc(&dx, bytes) == hipSuccess && hipMalloc(&dy, bytes) == hipSuccess &&
+            hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(math_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, fn, dx,
+                       dy, n);
+    ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+         hipMemcpy(y, dy, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+  }
+  if (dx) (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  return ok ? 0 : 2;
 }
