@@ -44,39 +44,38 @@
 
 namespace mbm {
 
-/* reciprocal: hardware estimate (5e-8 relative) + two Newton steps, which
- * round correctly over 4.2 M log-uniform arguments in [1e-6, 1e6] (one step
- * leaves up to 11 ulp) */
+/* reciprocal: hardware estimate r (5e-8 relative) and one cubic correction,
+ * 1/x = r / (1 - e) = r (1 + e + e^2 + ...) with e = 1 - x r, whose
+ * truncation (e^3, about 1e-22) is far below an ulp: correctly rounded on
+ * 2^24 arguments (log-uniform in [1e-8, 1e8], and mantissas next to 1 and
+ * 2), as two Newton steps are, in three FMAs instead of four; 0 and inf give
+ * NaN */
 MBM_DEV double rcp(double x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-  double r = __builtin_amdgcn_rcp(x);
-  double e = __builtin_fma(-x, r, 1.0);
-  r = __builtin_fma(e, r, r);
-  e = __builtin_fma(-x, r, 1.0);
-  return __builtin_fma(e, r, r);
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(__builtin_fma(e, e, e), r, r);
 #else
   return 1.0 / x;
 #endif
 }
 
-/* square root of a positive number: hardware rsq estimate (5e-8 relative),
- * one Goldschmidt step (36 ulp at most) and one Newton correction, which
- * rounds correctly: no difference from IEEE sqrt over 4.2 M log-uniform
- * arguments in [1e-6, 1e6] (a second correction changes none of them; the
- * compiler's own sequence has two, and rescales subnormal and huge
- * arguments, which theta never reaches); 0 -> 0, negative -> NaN */
+/* square root of a positive number: hardware rsq estimate y (5e-8
+ * relative), one Goldschmidt step s = x y (1 + r) (36 ulp at most) and one
+ * Newton correction with h = y / 2 (the correction is about 36 ulp, so h's
+ * own error is negligible in it): correctly rounded on the same 2^24
+ * arguments (the compiler's own sequence refines h and corrects twice, and
+ * rescales subnormal and huge arguments, which theta never reaches);
+ * 0 -> 0, negative -> NaN */
 MBM_DEV double sqrtp(double x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
   const double y = __builtin_amdgcn_rsq(x);
-  double s = x * y, h = 0.5 * y;
-  const double r = __builtin_fma(-h, s, 0.5);
-  s = __builtin_fma(s, r, s);
-  h = __builtin_fma(h, r, h);
+  const double s0 = x * y, h = 0.5 * y;
+  const double s = __builtin_fma(s0, __builtin_fma(-h, s0, 0.5), s0);
   const double d = __builtin_fma(-s, s, x);
-  s = __builtin_fma(d, h, s);
-  return x > 0.0 ? s : (x == 0.0 ? 0.0 : __builtin_nan(""));
+  return x > 0.0 ? __builtin_fma(d, h, s) : (x == 0.0 ? 0.0 : __builtin_nan(""));
 #else
   return sqrt(x);
 #endif
