@@ -1,6 +1,6 @@
 """Compact fixtures of the full-size runs, from the reference build.
 
-    python tests/golden/make_long_fixtures.py square4_1024 [lshape_500] [--run]
+    python tests/golden/make_long_fixtures.py square4_1024 [lshape_500] [lshape_2000] [--run]
 
 A full-size run of the reference (oracle/_ref/ref_harness steplog, OpenMP
 over blocks) is too large to keep (305 MB of FP64 state for 4 x 1024^2), so
@@ -14,7 +14,9 @@ what is kept of it is:
             (block, 'row'|'col', index, values (n, 9))
 written to tests/golden/long/<name>.npz with the deck and step limit used.
 
---run runs the reference first (hours for square4 to t = 0.3); without it
+--run runs the reference first (hours for square4 to t = 0.3 and for 2000
+steps of the L-shape; its run to t = 0.6 would take more than 12 hours on 3
+cores); without it
 the .log/.bin of a previous steplog run are read from --dir (default
 _work/long, not part of the repository).
 """
@@ -30,6 +32,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 RUNS = {
     "square4_1024": ("multiblockmomentsfv_amd/decks/square4_1024.txt", 1000000, 4),
     "lshape_500": ("multiblockmomentsfv_amd/decks/lshape_2048.txt", 500, 3),
+    "lshape_2000": ("multiblockmomentsfv_amd/decks/lshape_2048.txt", 2000, 3),
 }
 
 
