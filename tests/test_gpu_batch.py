@@ -134,3 +134,28 @@ def test_full_size_sweep_subset_equals_single_runs(mbm):
     for i in range(0, 256, 51):
         assert all(np.isfinite(b).all() for b in bat.download(i))
     bat.close()
+
+
+def test_batch_of_boundary_decks_equals_single_runs(mbm):
+    """Decks with walls, Dirichlet edges, periodic sync and an L-shape in one
+    batch (two groups on two streams): the stage kernels' ghost copies and
+    the per-group fill of the scenario (Dirichlet) ghosts leave every deck
+    bit-identical to its run alone."""
+    from conftest import deck_path
+    decks = []
+    for name in ("mixed_bc", "box_small", "periodic", "lshape_small", "mixed_bc"):
+        deck = mbm.read_deck(deck_path(name))
+        decks.append(dict(blocks=deck.blocks, kn=deck.kn, cfl=deck.cfl))
+    bat = mbm.Batch(decks)
+    bat.init()
+    ns, nr = bat.run(maxsteps=12)
+    assert (ns == 12).all()
+    for i, d in enumerate(decks):
+        eng = mbm.Engine(d["blocks"], kn=d["kn"])
+        eng.init()
+        t, n, rej, _ = eng.steps(0.0, 1e30, d["cfl"], maxsteps=12)
+        assert n == 12 and rej == nr[i] and t == bat.t[i]
+        for g, r in zip(bat.download(i), eng.download(0)):
+            assert g.tobytes() == r.tobytes(), f"deck {i}"
+        eng.close()
+    bat.close()
