@@ -107,6 +107,13 @@ int resolve_scratch(int dev)
 #define MBM_STAGE2_REVERSE 1
 #endif
 
+/* stage 2 reads DATA for the rows the tile stores only (MBM_DATA_OWN_ROWS=1):
+ * the march's last projections, two rows past the tile, need no initial
+ * value */
+#ifndef MBM_DATA_OWN_ROWS
+#define MBM_DATA_OWN_ROWS 1
+#endif
+
 /* the CFL parts of one tile: face-speed and production maxima (NaN
  * ignored), and 1 + the scan position of its last NaN production CFL */
 struct TileOut {
@@ -270,10 +277,13 @@ __device__ __forceinline__ int tile_pass(const StageArgs &A, int tix, double (*s
     for (int n = 0; n < NV; n++) acc[n] += from_left(F[n]) - F[n];
   };
   /* input rows, clamped to the block (row 0 and ly-1 are ghost ring 3) */
+  /* the stage input of a row, and in stage 2 its DATA, which only the
+   * initial values of the tile's own rows use (MBM_DATA_OWN_ROWS) */
   auto fetch = [&](int row, double *Ud, double *U0d) {
     const unsigned roff = cin + rowb(row_of(row));
     load9b(rin, pb, roff, Ud);
-    if constexpr (STAGE == 2) load9b(rin0, pb, roff, U0d);
+    if constexpr (STAGE == 2)
+      if (!MBM_DATA_OWN_ROWS || (row >= T.r0 && row <= T.r1)) load9b(rin0, pb, roff, U0d);
   };
   /* one row of the block: the fast face states, or the reference's
    * (slow), of the row with primitives pc between pm (below) and pp; zlo,
