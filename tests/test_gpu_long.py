@@ -7,6 +7,8 @@ state and FP64 rows and columns through the initial jump:
     square4_1024  4 x 1024^2, Reg13, Kn 0.01, CFL 0.9, harmonic, to t = 0.3
     lshape_500    L-shape 2048x1024 / 1024^2 / 1024x2048, minmod, Kn 0.05,
                   the first 500 steps of the run to t = 0.6
+    lshape_2000   the first 2000 steps of the same run (the whole run takes
+                  the reference more than 12 hours on 3 cores)
 The GPU runs the same steps. Bar: the same number of attempts and the same
 accept/reject sequence; dt, CFL and t of every attempt, the statistics and
 the lines within LONG_TOL relative (per field: max |a - b| / max |b|).
@@ -18,6 +20,16 @@ relative, after the 3432 steps of square4_1024), and tout - t turns that
 absolute difference into a relative one t/dt times larger in the clamped dt
 and in the CFL that is proportional to it. Those attempts are held to
 LONG_TOL in units of t: |d dt| <= LONG_TOL t, |d CFL| / CFL <= LONG_TOL t / dt.
+
+The L-shape flow amplifies rounding differences after about 600 steps
+(t ~ 0.04): the CFL logs of the reference and of this engine agree to 1e-15
+up to attempt 600, then their difference grows by about 10 % a step (1e-13 at
+attempt 649, 1e-10 at 694, 1e-6 at 776). Two builds of this engine whose
+arithmetic differs only in rounding (square roots and reciprocals refined
+differently, a regrouped initial value) part from each other in the same way,
+from attempt 602. So LONG_TOL holds for the first STRICT_ATTEMPTS attempts of
+such a run; the rest of it, and the final state, are held to TRAJECTORY_TOL:
+the same trajectory, to the size that rounding differences reach in it.
 """
 import os
 
@@ -31,6 +43,8 @@ pytestmark = pytest.mark.gpu
 LONG = os.path.join(ROOT, "tests", "golden", "long")
 NAMES = sorted(f[:-4] for f in os.listdir(LONG) if f.endswith(".npz")) if os.path.isdir(LONG) else []
 LONG_TOL = 1e-12
+TRAJECTORY_TOL = {"log": 1e-3, "stats": 1e-2, "lines": 1e-3}
+STRICT_ATTEMPTS = {"lshape_2000": 600}
 
 
 def rel(a, b):
@@ -56,8 +70,16 @@ def test_full_length_run_matches_the_reference(mbm, name):
     assert np.array_equal(mine[:, 3], log[:, 3]), "accept/reject sequence differs"
     clamped = np.isin(log[:, 2], np.asarray(deck.tout, dtype=np.float64)) & (log[:, 3] == 1)
     free = ~clamped
+    strict = STRICT_ATTEMPTS.get(name, len(log))
+    if strict < len(log):
+        early = slice(0, strict)
+        d_early = max(rel(mine[early, c], log[early, c]) for c in range(3))
+        d_all = max(rel(mine[:, c], log[:, c]) for c in range(3))
+        print(f"  attempts 0..{strict - 1}: rel diff {d_early:.2e}; whole run {d_all:.2e}")
+        assert d_early <= LONG_TOL and d_all <= TRAJECTORY_TOL["log"]
+        free[strict:] = False
     drift = {"cfl": rel(mine[free, 0], log[free, 0]), "dt": rel(mine[free, 1], log[free, 1]),
-             "t": rel(mine[:, 2], log[:, 2])}
+             "t": rel(mine[:strict, 2], log[:strict, 2])}
     for i in np.flatnonzero(clamped):
         ddt = abs(mine[i, 1] - log[i, 1]) / log[i, 2]
         dcfl = abs(mine[i, 0] - log[i, 0]) / log[i, 0] * log[i, 1] / log[i, 2]
@@ -93,4 +115,7 @@ def test_full_length_run_matches_the_reference(mbm, name):
     print(f"  final state: statistics {s_err:.2e}, rows/columns through the jump {l_err:.2e}")
     eng.close()
     assert max(drift.values()) <= LONG_TOL
-    assert s_err <= LONG_TOL and l_err <= LONG_TOL
+    if strict < len(log):
+        assert s_err <= TRAJECTORY_TOL["stats"] and l_err <= TRAJECTORY_TOL["lines"]
+    else:
+        assert s_err <= LONG_TOL and l_err <= LONG_TOL
